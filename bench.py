@@ -1,0 +1,249 @@
+"""Benchmark: batched real FFT round trip on one MI355X (drop-in C ABI, HIP kernels).
+
+Workload (one step): CkFftRealForwardBatch then CkFftRealInverseBatch over 256
+independent signals of 65536 real FP32 samples (iid N(0,1), seeded), inputs resident
+in HBM before timing starts.  The value is whole-step throughput in GFLOP/s with the
+2.5*N*log2(N) convention for real transforms; each launch's duration is taken with HIP
+events on the launch stream, which gives the roofline fraction against 8 TB/s.
+
+Also reported on the same line:
+  * plain: the same signals through the one-signal-per-call drop-in entry points
+    (CkFftRealForward / CkFftRealInverse with device pointers, synchronous);
+  * cpu_baseline: the reference's CPU path (oracle/_ref, built from the reference's
+    own sources; the CPU restatement if that is absent) timed on this host's cores on a
+    bounded sample of the same workload;
+  * parity: a sample of the benchmarked outputs against the CPU oracle and FP64.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FFT GFLOP/s (5N·log2N) & % HBM roofline, batched vs plain; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+N_REAL = 65536
+BATCH = 256
+
+
+def real_flops(n: int) -> float:
+    return 2.5 * n * math.log2(n)
+
+
+def real_bytes(n: int) -> int:
+    # one read of the input and one write of the output, per transform
+    return 4 * n + 8 * (n // 2 + 1)
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def cpu_baseline(n: int, seconds: float = 10.0):
+    """Reference CPU path on this host: all cores, bounded sample, GFLOP/s."""
+    import oracle as O
+
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    sample = 64
+    x = O.normal01(0x5EED, sample * n).reshape(sample, n)
+    use_ref = O.Reference.available()
+    if use_ref:
+        R = O.Reference()
+        ctx = R.init(n, 3)
+        lib = R.lib
+        yb = np.empty((sample, 2 * (n // 2 + 1)), np.float32)
+        zb = np.empty((sample, n), np.float32)
+        tmps = [np.empty(2 * (n // 2 + 1), np.float32) for _ in range(threads)]
+
+        def work(t):
+            for b in range(t, sample, threads):
+                lib.CkFftRealForward(ctx, n, x[b].ctypes.data, yb[b].ctypes.data)
+                lib.CkFftRealInverse(ctx, n, yb[b].ctypes.data, zb[b].ctypes.data, tmps[t].ctypes.data)
+
+        def one_pass():
+            ts = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+            for th in ts:
+                th.start()
+            for th in ts:
+                th.join()
+    else:
+        o = O.Restatement(n, 3)
+
+        def one_pass():
+            y = o.real_forward(x, threads)
+            o.real_inverse(y, n, threads)
+
+    one_pass()  # warm
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        one_pass()
+        passes += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    if use_ref:
+        R.shutdown(ctx)
+    gflops = passes * sample * 2 * real_flops(n) / el / 1e9
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{passes} passes x {sample} signals x (real fwd + real inv), n={n}, "
+                      f"{el:.1f} s wall, {threads} threads sharing one context",
+            "cpu": cpu}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-signal leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the transform path "
+              "runs on one GPU and this build has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import ckfft_amd as C
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, B = N_REAL, BATCH
+    half = n // 2 + 1
+
+    rng_x = None
+    import oracle as O  # seeded input generator and the parity checker
+    rng_x = O.normal01(0xBE7C4, B * n).reshape(B, n)
+    x = torch.from_numpy(rng_x).to(dev)
+    y = torch.empty((B, 2 * half), dtype=torch.float32, device=dev)
+    z = torch.empty((B, n), dtype=torch.float32, device=dev)
+    ctx = C.Context(n, C.kCkFftDirection_Both)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        assert ctx.real_forward_batch(n, B, x, y, stream) == 1
+        if ev is not None:
+            ev[1].record(stream)
+        assert ctx.real_inverse_batch(n, B, y, z, stream) == 1
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # whole-step timing: K steps between two events, synchronised on both sides
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    ms_per_step = total_ms / args.steps
+    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    inv_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+
+    flops_step = B * 2 * real_flops(n)
+    value = flops_step / (ms_per_step * 1e-3) / 1e9
+    bytes_launch = B * real_bytes(n)
+    dom_name, dom_ms = ("real_inverse", inv_ms) if inv_ms >= fwd_ms else ("real_forward", fwd_ms)
+    achieved = bytes_launch / (dom_ms * 1e-3) / 1e9
+    workload = f"real_fft_{n}_x{B}"
+    traffic = load_traffic(workload)
+    roofline = {
+        "bound": "hbm", "kernel": f"fft_lds_kernel<15,5,1024,{dom_name}>",
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": (traffic or {}).get(dom_name),
+        "algorithmic_bytes_per_launch": bytes_launch,
+        "launch_ms": {"real_forward": round(fwd_ms, 5), "real_inverse": round(inv_ms, 5)},
+        "frac_per_launch": {"real_forward": round(bytes_launch / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "real_inverse": round(bytes_launch / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+
+    # parity of the benchmarked outputs (sample of signals) against the CPU oracle and FP64
+    idx = [0, 97, B - 1]
+    o = O.Restatement(n, 3)
+    yh = y.cpu().numpy()
+    zh = z.cpu().numpy()
+    parity = {
+        "real_forward_vs_ref": O.split_norm_err(yh[idx], o.real_forward(rng_x[idx], 8)),
+        "real_forward_relL2_fp64": O.rel_l2(yh[idx], O.interleave(O.fp64_real_forward(rng_x[idx]))),
+        "real_inverse_vs_ref": O.split_norm_err(zh[idx], o.real_inverse(yh[idx], n, 8)),
+        "real_inverse_relL2_fp64": O.rel_l2(zh[idx], O.fp64_real_inverse(yh[idx], n)),
+        "gate": "vs_ref <= 1e-5 (max|d| / (||ref||/sqrt(len))), relL2_fp64 <= 1e-6",
+    }
+    parity = {k: (float(f"{v:.3e}") if isinstance(v, float) else v) for k, v in parity.items()}
+
+    plain = None
+    if not args.no_plain:
+        tmp = torch.empty(2 * half, dtype=torch.float32, device=dev)
+        for b in range(4):
+            ctx.real_forward(n, x[b], y[b])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for b in range(B):
+            assert ctx.real_forward(n, x[b], y[b]) == 1
+            assert ctx.real_inverse(n, y[b], z[b], tmp) == 1
+        el = time.perf_counter() - t0
+        plain_gflops = flops_step / el / 1e9
+        plain = {"value": round(plain_gflops, 2), "unit": "GFLOP/s", "ms_per_step": round(el * 1e3, 3),
+                 "calls": 2 * B, "batched_over_plain": round(value / plain_gflops, 1)}
+
+    cpu = None if args.no_cpu else cpu_baseline(n)
+    ctx.close()
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "n": n, "batch": B, "kinds": "real forward + real inverse",
+                   "flops_convention": "2.5*N*log2(N) per real transform",
+                   "input": "iid N(0,1) float32, SplitMix64/Box-Muller seed 0xBE7C4"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "parity": parity,
+        "device": torch.cuda.get_device_name(dev),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,144 @@
+"""ckfft_amd -- MI355X-native drop-in for ckfft's transform path.
+
+The product is the C-ABI library ``libckfft_amd.so`` (declared in ``include/ckfft/``):
+the reference's six entry points (``CkFftInit``, ``CkFftShutdown``,
+``CkFftComplexForward/Inverse``, ``CkFftRealForward/Inverse``) plus batched,
+stream-ordered extensions.  This module is a thin ctypes binding of that library,
+with the same names, argument meanings and 0/1/NULL error behaviour, so that Python
+callers and the tests drive exactly the code a C caller would.
+
+There is no CPU fallback: importing this module raises if the HIP library has not
+been built (``python ckfft_amd/build.py``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+__all__ = [
+    "LIB_PATH", "lib", "CkFftComplex", "kCkFftDirection_Forward", "kCkFftDirection_Inverse",
+    "kCkFftDirection_Both", "CkFftInit", "CkFftShutdown", "CkFftComplexForward", "CkFftComplexInverse",
+    "CkFftRealForward", "CkFftRealInverse", "CkFftComplexForwardBatch", "CkFftComplexInverseBatch",
+    "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "Context",
+    "EXPORTED_SYMBOLS",
+]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python ckfft_amd/build.py` "
+                      "(there is no CPU fallback)")
+lib = ctypes.CDLL(LIB_PATH)
+
+
+class CkFftComplex(ctypes.Structure):
+    """``{float real; float imag;}`` -- inc/ckfft/ckfft.h:10-15 of the reference."""
+    _fields_ = [("real", ctypes.c_float), ("imag", ctypes.c_float)]
+
+
+kCkFftDirection_Forward = 1
+kCkFftDirection_Inverse = 2
+kCkFftDirection_Both = 3
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+_SIGS = {
+    "CkFftInit": (_vp, [_i, _i, _vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "CkFftShutdown": (None, [_vp]),
+    "CkFftComplexForward": (_i, [_vp, _i, _vp, _vp]),
+    "CkFftComplexInverse": (_i, [_vp, _i, _vp, _vp]),
+    "CkFftRealForward": (_i, [_vp, _i, _vp, _vp]),
+    "CkFftRealInverse": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "CkFftComplexForwardBatch": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "CkFftComplexInverseBatch": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "CkFftRealForwardBatch": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "CkFftRealInverseBatch": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "CkFftGetTwiddles": (_i, [_vp, _i, _vp]),
+    "CkFftGpuVersion": (ctypes.c_char_p, []),
+}
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+    globals()[_name] = _fn
+
+
+def _ptr(x):
+    """Accept a raw address, None, a ctypes object/array, or a torch tensor."""
+    if x is None or isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    return ctypes.cast(x, ctypes.c_void_p).value
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    return getattr(s, "cuda_stream", s)
+
+
+class Context:
+    """Owns a ``CkFftContext*``; methods mirror the C calls and return their 0/1 result.
+
+    Buffers may be torch tensors (device or host), numpy arrays or raw addresses.
+    """
+
+    def __init__(self, n_max: int, direction: int = kCkFftDirection_Both):
+        self.n_max = n_max
+        self.direction = direction
+        self.handle = CkFftInit(n_max, direction, None, None)
+        if not self.handle:
+            raise RuntimeError(f"CkFftInit({n_max}, {direction}) failed (invalid size/direction or no GPU)")
+
+    def close(self):
+        if self.handle:
+            CkFftShutdown(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # single-signal, synchronous (the reference's calls)
+    def complex_forward(self, n, x, y) -> int:
+        return CkFftComplexForward(self.handle, n, _ptr(x), _ptr(y))
+
+    def complex_inverse(self, n, x, y) -> int:
+        return CkFftComplexInverse(self.handle, n, _ptr(x), _ptr(y))
+
+    def real_forward(self, n, x, y) -> int:
+        return CkFftRealForward(self.handle, n, _ptr(x), _ptr(y))
+
+    def real_inverse(self, n, y, x, tmp) -> int:
+        return CkFftRealInverse(self.handle, n, _ptr(y), _ptr(x), _ptr(tmp))
+
+    # batched, stream-ordered (device memory)
+    def complex_forward_batch(self, n, count, x, y, stream=None) -> int:
+        return CkFftComplexForwardBatch(self.handle, n, count, _ptr(x), _ptr(y), _stream(stream))
+
+    def complex_inverse_batch(self, n, count, x, y, stream=None) -> int:
+        return CkFftComplexInverseBatch(self.handle, n, count, _ptr(x), _ptr(y), _stream(stream))
+
+    def real_forward_batch(self, n, count, x, y, stream=None) -> int:
+        return CkFftRealForwardBatch(self.handle, n, count, _ptr(x), _ptr(y), _stream(stream))
+
+    def real_inverse_batch(self, n, count, y, x, stream=None) -> int:
+        return CkFftRealInverseBatch(self.handle, n, count, _ptr(y), _ptr(x), _stream(stream))
+
+    def twiddles(self, L: int, out) -> int:
+        return CkFftGetTwiddles(self.handle, L, _ptr(out))

@@ -1,0 +1,273 @@
+// ckfft_api.cpp -- the C-ABI boundary: ckfft's six entry points plus the batched
+// extensions, backed by the HIP kernels.
+//
+// Argument validation follows the reference's C API (src/ckfft/ckfft.cpp:14-114) rule
+// for rule and returns the same 0 / 1 / NULL values; the context's size query and
+// caller-buffer mode follow src/ckfft/context.cpp:45-91.  Differences, all deliberate:
+//  * n <= 0 is rejected (the reference's unsigned power-of-two test lets INT_MIN in);
+//  * a context pointer is also checked for a live magic value;
+//  * 0 is also returned if the GPU reports an error for the call;
+//  * the size written in query mode covers only the host part of the context, since
+//    the twiddle tables live in device memory the library allocates itself.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "ckfft/ckfft.h"
+#include "ckfft/ckfft_gpu.h"
+#include "ckfft_internal.h"
+
+using namespace ckfft_gpu;
+
+namespace {
+
+bool pow2_positive(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+size_t host_context_bytes() { return (sizeof(_CkFftContext) + 7) & ~size_t(7); }
+
+bool context_ok(const CkFftContext* c, int need_direction)
+{
+    return c && c->magic == kContextMagic && (c->direction & need_direction);
+}
+
+// src/ckfft/ckfft.cpp:42-49 / 84-91: size, pointer and aliasing rules.
+bool size_ok(const CkFftContext* c, int n) { return pow2_positive(n) && n <= c->max_count; }
+
+size_t in_bytes(Kind k, int n)
+{
+    switch (k) {
+    case kRealForward: return sizeof(float) * (size_t)n;
+    case kRealInverse: return sizeof(CkFftComplex) * ((size_t)n / 2 + 1);
+    default: return sizeof(CkFftComplex) * (size_t)n;
+    }
+}
+
+size_t out_bytes(Kind k, int n)
+{
+    switch (k) {
+    case kRealForward: return sizeof(CkFftComplex) * ((size_t)n / 2 + 1);
+    case kRealInverse: return sizeof(float) * (size_t)n;
+    default: return sizeof(CkFftComplex) * (size_t)n;
+    }
+}
+
+// Device memory (hipMalloc, torch) runs in place; anything else is staged.
+bool is_device_pointer(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // plain pageable host memory is unknown to HIP
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int fail(hipError_t) { return 0; }
+
+// One synchronous transform, as every drop-in call is.
+int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
+{
+    const int log2n = ilog2(n);
+    if (log2n > max_log2n(kind)) return 0;
+    const bool din = is_device_pointer(in), dout = is_device_pointer(out);
+    hipStream_t s = nullptr;
+    if (din && dout) {
+        hipError_t e = launch_transform(kind, log2n, 1, in, out, c->d_pyramid, c->log_pyramid, s);
+        if (e != hipSuccess) return fail(e);
+        return hipStreamSynchronize(s) == hipSuccess ? 1 : 0;
+    }
+    // Host buffers: stage through the context's device buffer.
+    const size_t ib = in_bytes(kind, n), ob = out_bytes(kind, n);
+    const size_t ib_al = (ib + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> lock(*c->staging_mutex);
+    const size_t need = ib_al + ob;
+    if (c->staging_bytes < need) {
+        if (c->d_staging) (void)hipFree(c->d_staging);
+        c->d_staging = nullptr;
+        c->staging_bytes = 0;
+        if (hipMalloc(&c->d_staging, need) != hipSuccess) return 0;
+        c->staging_bytes = need;
+    }
+    char* st = (char*)c->d_staging;
+    const void* src = din ? in : (const void*)st;
+    void* dst = dout ? out : (void*)(st + ib_al);
+    if (!din && hipMemcpyAsync(st, in, ib, hipMemcpyHostToDevice, s) != hipSuccess) return 0;
+    if (launch_transform(kind, log2n, 1, src, dst, c->d_pyramid, c->log_pyramid, s) != hipSuccess) return 0;
+    if (!dout && hipMemcpyAsync(out, dst, ob, hipMemcpyDeviceToHost, s) != hipSuccess) return 0;
+    return hipStreamSynchronize(s) == hipSuccess ? 1 : 0;
+}
+
+int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void* out, void* stream)
+{
+    if (!size_ok(c, n) || count < 0) return 0;
+    if (!in || !out || in == out) return 0;
+    const int log2n = ilog2(n);
+    if (log2n > max_log2n(kind)) return 0;
+    if (count == 0) return 1;
+    if (!is_device_pointer(in) || !is_device_pointer(out)) return 0;
+    hipError_t e = launch_transform(kind, log2n, count, in, out, c->d_pyramid, c->log_pyramid,
+                                    (hipStream_t)stream);
+    return e == hipSuccess ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// src/ckfft/ckfft.cpp:14-34 and src/ckfft/context.cpp:45-135
+CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* bufSize)
+{
+    if (!pow2_positive(nMax)) return nullptr;
+    if (direction != kCkFftDirection_Forward && direction != kCkFftDirection_Inverse &&
+        direction != kCkFftDirection_Both)
+        return nullptr;
+    if (buf && !bufSize) return nullptr;
+    const size_t need = host_context_bytes();
+    if (bufSize && (!buf || *bufSize < need)) {
+        *bufSize = need;
+        return nullptr;
+    }
+
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const int log_pyr = std::max(1, std::min(ilog2(nMax), kLogMaxPyramid));
+    float2* pyr = nullptr;
+    if (hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)) != hipSuccess) return nullptr;
+    if (launch_twiddle_pyramid(pyr, log_pyr, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipFree(pyr);
+        return nullptr;
+    }
+
+    void* mem = buf ? buf : std::malloc(need);
+    if (!mem) {
+        (void)hipFree(pyr);
+        return nullptr;
+    }
+    std::mutex* mtx = new (std::nothrow) std::mutex();
+    if (!mtx) {
+        (void)hipFree(pyr);
+        if (!buf) std::free(mem);
+        return nullptr;
+    }
+    CkFftContext* c = new (mem) CkFftContext();
+    c->magic = kContextMagic;
+    c->max_count = nMax;
+    c->direction = (int)direction;
+    c->own_host = buf ? 0 : 1;
+    c->device = dev;
+    c->log_pyramid = log_pyr;
+    c->d_pyramid = pyr;
+    c->staging_mutex = mtx;
+    c->d_staging = nullptr;
+    c->staging_bytes = 0;
+    return c;
+}
+
+// src/ckfft/ckfft.cpp:36-53
+int CkFftRealForward(CkFftContext* context, int n, const float* input, CkFftComplex* output)
+{
+    if (!context_ok(context, kCkFftDirection_Forward)) return 0;
+    if (!size_ok(context, n)) return 0;
+    if (!input || !output || (const void*)input == (const void*)output) return 0;
+    return run_plain(context, kRealForward, n, input, output);
+}
+
+// src/ckfft/ckfft.cpp:55-76 (tmpBuf is checked first, as there)
+int CkFftRealInverse(CkFftContext* context, int n, const CkFftComplex* input, float* output, CkFftComplex* tmpBuf)
+{
+    if (!tmpBuf) return 0;
+    if (!context_ok(context, kCkFftDirection_Inverse)) return 0;
+    if (!size_ok(context, n)) return 0;
+    if (!input || !output || (const void*)input == (const void*)output) return 0;
+    return run_plain(context, kRealInverse, n, input, output);
+}
+
+// src/ckfft/ckfft.cpp:78-95
+int CkFftComplexForward(CkFftContext* context, int n, const CkFftComplex* input, CkFftComplex* output)
+{
+    if (!context_ok(context, kCkFftDirection_Forward)) return 0;
+    if (!size_ok(context, n)) return 0;
+    if (!input || !output || input == output) return 0;
+    return run_plain(context, kComplexForward, n, input, output);
+}
+
+// src/ckfft/ckfft.cpp:97-114
+int CkFftComplexInverse(CkFftContext* context, int n, const CkFftComplex* input, CkFftComplex* output)
+{
+    if (!context_ok(context, kCkFftDirection_Inverse)) return 0;
+    if (!size_ok(context, n)) return 0;
+    if (!input || !output || input == output) return 0;
+    return run_plain(context, kComplexInverse, n, input, output);
+}
+
+// src/ckfft/ckfft.cpp:116-119 and src/ckfft/context.cpp:137-143
+void CkFftShutdown(CkFftContext* context)
+{
+    if (!context || context->magic != kContextMagic) return;
+    (void)hipFree(context->d_pyramid);
+    if (context->d_staging) (void)hipFree(context->d_staging);
+    delete context->staging_mutex;
+    const int own = context->own_host;
+    context->magic = 0;
+    context->~CkFftContext();
+    if (own) std::free(context);
+}
+
+int CkFftComplexForwardBatch(CkFftContext* context, int n, int count, const CkFftComplex* input,
+                             CkFftComplex* output, void* stream)
+{
+    if (!context_ok(context, kCkFftDirection_Forward)) return 0;
+    return run_batch(context, kComplexForward, n, count, input, output, stream);
+}
+
+int CkFftComplexInverseBatch(CkFftContext* context, int n, int count, const CkFftComplex* input,
+                             CkFftComplex* output, void* stream)
+{
+    if (!context_ok(context, kCkFftDirection_Inverse)) return 0;
+    return run_batch(context, kComplexInverse, n, count, input, output, stream);
+}
+
+int CkFftRealForwardBatch(CkFftContext* context, int n, int count, const float* input, CkFftComplex* output,
+                          void* stream)
+{
+    if (!context_ok(context, kCkFftDirection_Forward)) return 0;
+    return run_batch(context, kRealForward, n, count, input, output, stream);
+}
+
+int CkFftRealInverseBatch(CkFftContext* context, int n, int count, const CkFftComplex* input, float* output,
+                          void* stream)
+{
+    if (!context_ok(context, kCkFftDirection_Inverse)) return 0;
+    return run_batch(context, kRealInverse, n, count, input, output, stream);
+}
+
+int CkFftGetTwiddles(CkFftContext* context, int L, CkFftComplex* out)
+{
+    if (!context || context->magic != kContextMagic || !out) return 0;
+    if (!pow2_positive(L) || ilog2(L) > context->log_pyramid) return 0;
+    return hipMemcpy(out, context->d_pyramid + L, sizeof(float2) * (size_t)L, hipMemcpyDeviceToHost) == hipSuccess
+               ? 1
+               : 0;
+}
+
+const char* CkFftGpuVersion(void)
+{
+    return "ckfft_amd 0.1 (gfx950 HIP; drop-in for ckfft 1.3 C API)";
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// fft_device.h -- device-side complex arithmetic and small DFT butterflies (gfx950).
+//
+// Sign convention: forward transforms use exp(-2 pi i k n / N), inverse exp(+...),
+// unnormalised, as in the reference's complex transform
+// (src/ckfft/fft_default.cpp:204-256 states the radix-4 form of the same DFT).
+// Twiddles come from the device table (FP64-generated, rounded to FP32); the only
+// constants here are the radix-8/16 internal rotations, written to FP32 precision.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ckfft_gpu {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Forward twiddles are stored; the inverse transform uses their conjugates.
+template <bool INV>
+__device__ __forceinline__ float2 dirw(float2 w) { return INV ? make_float2(w.x, -w.y) : w; }
+
+// a * (-i) for forward, a * (+i) for inverse: the W_4^1 rotation.
+template <bool INV>
+__device__ __forceinline__ float2 rot4(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// a * W_8^1 (forward: exp(-i pi/4)).
+template <bool INV>
+__device__ __forceinline__ float2 rot8(float2 a)
+{
+    const float h = 0.70710678118654752f;
+    return INV ? make_float2((a.x - a.y) * h, (a.x + a.y) * h)
+               : make_float2((a.x + a.y) * h, (a.y - a.x) * h);
+}
+
+// a * W_8^3 (forward: exp(-3 i pi/4)).
+template <bool INV>
+__device__ __forceinline__ float2 rot8_3(float2 a)
+{
+    const float h = 0.70710678118654752f;
+    return INV ? make_float2(-(a.x + a.y) * h, (a.x - a.y) * h)
+               : make_float2((a.y - a.x) * h, -(a.x + a.y) * h);
+}
+
+// a * W_16^m for the exponents the 4x4 radix-16 needs (m = n2*k1 in {1,2,3,4,6,9}).
+template <bool INV, int M>
+__device__ __forceinline__ float2 rot16(float2 a)
+{
+    const float c = 0.92387953251128674f;   // cos(pi/8)
+    const float s = 0.38268343236508977f;   // sin(pi/8)
+    if constexpr (M == 0) return a;
+    else if constexpr (M == 2) return rot8<INV>(a);
+    else if constexpr (M == 4) return rot4<INV>(a);
+    else if constexpr (M == 6) return rot8_3<INV>(a);
+    else {
+        // W_16^M = (cos t, -sin t), t = 2 pi M / 16 for forward; conjugate for inverse
+        constexpr int m = M & 15;
+        float wc, ws;
+        if constexpr (m == 1) { wc = c;  ws = s; }
+        else if constexpr (m == 3) { wc = s;  ws = c; }
+        else if constexpr (m == 9) { wc = -c; ws = -s; }
+        else { static_assert(m == 1 || m == 3 || m == 9, "unsupported W16 exponent"); }
+        const float wi = INV ? ws : -ws;
+        return make_float2(a.x * wc - a.y * wi, a.x * wi + a.y * wc);
+    }
+}
+
+// ---- in-register DFTs on a[0..R-1] (natural order in, natural order out) -------
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a0, float2& a1)
+{
+    float2 t = a0;
+    a0 = cadd(t, a1);
+    a1 = csub(t, a1);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3)
+{
+    float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    float2 t2 = cadd(a1, a3), t3 = rot4<INV>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* a)
+{
+    // even/odd split: E = DFT4(a0,a2,a4,a6), O = DFT4(a1,a3,a5,a7)
+    dft4<INV>(a[0], a[2], a[4], a[6]);
+    dft4<INV>(a[1], a[3], a[5], a[7]);
+    float2 o1 = rot8<INV>(a[3]), o2 = rot4<INV>(a[5]), o3 = rot8_3<INV>(a[7]);
+    float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6], o0 = a[1];
+    a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+    a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+    a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+    a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* a)
+{
+    // n = n2 + 4 n1, k = k1 + 4 k2: column DFT4 over n1, twiddle W16^(n2 k1),
+    // row DFT4 over n2.
+    dft4<INV>(a[0], a[4], a[8], a[12]);
+    dft4<INV>(a[1], a[5], a[9], a[13]);
+    dft4<INV>(a[2], a[6], a[10], a[14]);
+    dft4<INV>(a[3], a[7], a[11], a[15]);
+    // b[n2][k1] sits in a[n2 + 4 k1]
+    a[5] = rot16<INV, 1>(a[5]);
+    a[9] = rot16<INV, 2>(a[9]);
+    a[13] = rot16<INV, 3>(a[13]);
+    a[6] = rot16<INV, 2>(a[6]);
+    a[10] = rot16<INV, 4>(a[10]);
+    a[14] = rot16<INV, 6>(a[14]);
+    a[7] = rot16<INV, 3>(a[7]);
+    a[11] = rot16<INV, 6>(a[11]);
+    a[15] = rot16<INV, 9>(a[15]);
+    // row DFT4 over n2 for each k1: inputs a[4 k1 + n2], outputs X[k1 + 4 k2]
+    dft4<INV>(a[0], a[1], a[2], a[3]);
+    dft4<INV>(a[4], a[5], a[6], a[7]);
+    dft4<INV>(a[8], a[9], a[10], a[11]);
+    dft4<INV>(a[12], a[13], a[14], a[15]);
+    // now a[4 k1 + k2] = X[k1 + 4 k2]: transpose the 4x4 index to natural order
+    float2 t;
+    t = a[1]; a[1] = a[4]; a[4] = t;
+    t = a[2]; a[2] = a[8]; a[8] = t;
+    t = a[3]; a[3] = a[12]; a[12] = t;
+    t = a[6]; a[6] = a[9]; a[9] = t;
+    t = a[7]; a[7] = a[13]; a[13] = t;
+    t = a[11]; a[11] = a[14]; a[14] = t;
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* a)
+{
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) {
+        dft2<INV>(a[0], a[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(a[0], a[1], a[2], a[3]);
+    } else if constexpr (R == 8) {
+        dft8<INV>(a);
+    } else {
+        static_assert(R == 16, "radix must be 1, 2, 4, 8 or 16");
+        dft16<INV>(a);
+    }
+}
+
+}  // namespace ckfft_gpu

@@ -1,0 +1,328 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracles.
+
+Gates (per transform, worst over the batch):
+  * vs the bit-exact CPU restatement of the reference (oracle.Restatement):
+        max|gpu - ref| <= 1e-5 * ||ref||_2 / sqrt(len(ref))
+    (the output-norm reading of "max abs diff <= 1e-5 ||x||_2 / sqrt(N)";
+    ckfft does not normalise, so the outputs carry the scale);
+  * vs the FP64 oracle: relative L2 error <= 1e-6 over the whole batch.
+Closed-form sizes (complex n <= 2, real n <= 4) must match the reference bit for bit.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_REF = 1e-5
+TOL_FP64 = 1e-6
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a HIP device"
+    return _t
+
+
+@pytest.fixture(scope="module")
+def C():
+    import ckfft_amd
+    return ckfft_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(C, torch):
+    c = C.Context(1 << 16, C.kCkFftDirection_Both)
+    yield c
+    c.close()
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda")
+
+
+def out_len(kind, n):
+    return {"cfwd": 2 * n, "cinv": 2 * n, "rfwd": 2 * (n // 2 + 1), "rinv": n}[kind]
+
+
+def run_batch(torch, ctx, kind, n, x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    count = x.shape[0]
+    xt = dev(torch, x)
+    yt = torch.full((count, out_len(kind, n)), float("nan"), dtype=torch.float32, device="cuda")
+    fn = {"cfwd": ctx.complex_forward_batch, "cinv": ctx.complex_inverse_batch,
+          "rfwd": ctx.real_forward_batch, "rinv": ctx.real_inverse_batch}[kind]
+    assert fn(n, count, xt, yt) == 1
+    torch.cuda.synchronize()
+    return yt.cpu().numpy()
+
+
+def oracle_out(kind, n, x, threads=8):
+    o = O.Restatement(max(n, 1), 3)
+    if kind in ("cfwd", "cinv"):
+        return o.complex(x, kind == "cinv", threads)
+    if kind == "rfwd":
+        return o.real_forward(x, threads)
+    return o.real_inverse(x, n, threads)
+
+
+def fp64_out(kind, n, x):
+    if kind in ("cfwd", "cinv"):
+        return O.interleave(O.fp64_complex(x, kind == "cinv"))
+    if kind == "rfwd":
+        return O.interleave(O.fp64_real_forward(x))
+    return O.fp64_real_inverse(x, n)
+
+
+def make_input(kind, n, count, seed):
+    if kind in ("cfwd", "cinv"):
+        return O.uniform_pm1(seed, count * 2 * n).reshape(count, 2 * n)
+    if kind == "rfwd":
+        return O.normal01(seed, count * n).reshape(count, n)
+    # real inverse: half spectra of real signals (what callers pass), as the oracle makes them
+    x = O.normal01(seed, count * n).reshape(count, n)
+    return O.Restatement(n, 3).real_forward(x)
+
+
+def check(kind, n, got, x, exact_small=True):
+    ref = oracle_out(kind, n, x)
+    assert np.all(np.isfinite(got)), (kind, n)
+    if exact_small and ((kind in ("cfwd", "cinv") and n <= 2) or (kind in ("rfwd", "rinv") and n <= 4)):
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (kind, n)
+        return
+    e_ref = O.split_norm_err(got, ref)
+    e64 = O.rel_l2(got, fp64_out(kind, n, x))
+    assert e_ref <= TOL_REF, (kind, n, e_ref)
+    assert e64 <= TOL_FP64, (kind, n, e64)
+
+
+# ------------------------------------------------------------------------- twiddles
+
+def test_twiddle_table_is_fp64_rounded(ctx, C):
+    for lg in range(0, 17):
+        L = 1 << lg
+        buf = np.zeros(2 * L, np.float32)
+        assert ctx.twiddles(L, buf) == 1
+        k = np.arange(L)
+        exact = O.exact_twiddle(k, L, -1)
+        want = O.interleave(exact).astype(np.float32)
+        err = np.abs(buf.astype(np.float64) - O.interleave(exact))
+        # correctly rounded FP64 -> FP32: within half an FP32 ulp of the exact value
+        assert float(err.max()) <= 2.0 ** -25, (L, float(err.max()))
+        assert np.mean(buf != want) < 1e-4, L
+        if L >= 4:
+            assert buf[2 * (L // 4)] == 0.0 and buf[2 * (L // 4) + 1] == -1.0
+
+
+# ------------------------------------------------------------------------- all sizes
+
+@pytest.mark.parametrize("kind", ["cfwd", "cinv"])
+@pytest.mark.parametrize("k", range(0, 16))
+def test_complex_sizes(torch, ctx, kind, k):
+    n = 1 << k
+    count = max(3, min(257, (1 << 16) // n))
+    x = make_input(kind, n, count, 1000 + 10 * k)
+    check(kind, n, run_batch(torch, ctx, kind, n, x), x)
+
+
+@pytest.mark.parametrize("k", range(0, 17))
+def test_real_forward_sizes(torch, ctx, k):
+    n = 1 << k
+    count = max(3, min(131, (1 << 17) // n))
+    x = make_input("rfwd", n, count, 2000 + k)
+    check("rfwd", n, run_batch(torch, ctx, "rfwd", n, x), x)
+
+
+@pytest.mark.parametrize("k", range(0, 17))
+def test_real_inverse_sizes(torch, ctx, k):
+    n = 1 << k
+    count = max(3, min(131, (1 << 17) // n))
+    x = make_input("rinv", n, count, 3000 + k)
+    check("rinv", n, run_batch(torch, ctx, "rinv", n, x), x)
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 9, 12, 15, 16])
+def test_real_inverse_arbitrary_spectra(torch, ctx, k):
+    """Inputs that are not half spectra of real data: Im Y[0] and Im Y[N/2] enter the
+    result exactly as in the reference's pre-pass."""
+    n = 1 << k
+    x = O.uniform_pm1(4000 + k, 3 * 2 * (n // 2 + 1)).reshape(3, -1)
+    check("rinv", n, run_batch(torch, ctx, "rinv", n, x), x)
+
+
+# ------------------------------------------------------------------------- API behaviour
+
+@pytest.mark.parametrize("kind,k", [("cfwd", 0), ("cinv", 1), ("cfwd", 2), ("cinv", 6), ("cfwd", 12),
+                                    ("rfwd", 2), ("rfwd", 3), ("rinv", 8), ("rfwd", 13), ("rinv", 16),
+                                    ("cfwd", 15)])
+def test_plain_call_equals_batch_bitwise(torch, ctx, kind, k):
+    """A signal's result does not depend on the batch it is in (same kernel, same bits)."""
+    n = 1 << k
+    x = make_input(kind, n, 5, 5000 + k)
+    batch = run_batch(torch, ctx, kind, n, x)
+    xt = dev(torch, x)
+    for b in range(x.shape[0]):
+        y = torch.full((out_len(kind, n),), float("nan"), dtype=torch.float32, device="cuda")
+        if kind == "cfwd":
+            ok = ctx.complex_forward(n, xt[b], y)
+        elif kind == "cinv":
+            ok = ctx.complex_inverse(n, xt[b], y)
+        elif kind == "rfwd":
+            ok = ctx.real_forward(n, xt[b], y)
+        else:
+            tmp = torch.empty(2 * (n // 2 + 1), dtype=torch.float32, device="cuda")
+            ok = ctx.real_inverse(n, xt[b], y, tmp)
+        assert ok == 1
+        assert np.array_equal(y.cpu().numpy().view(np.uint32), batch[b].view(np.uint32)), b
+
+
+@pytest.mark.parametrize("kind,k", [("cfwd", 10), ("cinv", 4), ("rfwd", 11), ("rinv", 12), ("rinv", 2)])
+def test_host_pointers_are_staged(torch, ctx, kind, k):
+    """Host buffers (the reference's only kind) work and give the device-path bits."""
+    n = 1 << k
+    x = make_input(kind, n, 1, 6000 + k)
+    want = run_batch(torch, ctx, kind, n, x)[0]
+    xh = np.ascontiguousarray(x[0])
+    y = np.full(out_len(kind, n), np.nan, np.float32)
+    if kind == "cfwd":
+        ok = ctx.complex_forward(n, xh, y)
+    elif kind == "cinv":
+        ok = ctx.complex_inverse(n, xh, y)
+    elif kind == "rfwd":
+        ok = ctx.real_forward(n, xh, y)
+    else:
+        ok = ctx.real_inverse(n, xh, y, np.zeros(2 * (n // 2 + 1), np.float32))
+    assert ok == 1
+    assert np.array_equal(y.view(np.uint32), want.view(np.uint32))
+
+
+def test_max_count_does_not_change_results(torch, C):
+    """Like the reference (float twiddle angle scaling by 2^j is exact), results are
+    identical for any nMax >= n."""
+    for kind, n in (("cfwd", 1024), ("rfwd", 4096), ("rinv", 256), ("cinv", 8)):
+        x = make_input(kind, n, 2, 7000 + n)
+        outs = []
+        for m in (n, 2 * n, 1 << 16):
+            with C.Context(m) as c:
+                outs.append(run_batch(torch, c, kind, n, x))
+        assert all(np.array_equal(o.view(np.uint32), outs[0].view(np.uint32)) for o in outs), (kind, n)
+
+
+def test_argument_validation_on_device(torch, C):
+    """src/ckfft/ckfft.cpp:36-114: 0 for every invalid call, 1 otherwise."""
+    x = torch.zeros(64, dtype=torch.float32, device="cuda")
+    y = torch.zeros(64, dtype=torch.float32, device="cuda")
+    t = torch.zeros(64, dtype=torch.float32, device="cuda")
+    with C.Context(16, C.kCkFftDirection_Forward) as f, C.Context(16, C.kCkFftDirection_Inverse) as i:
+        assert f.complex_forward(16, x, y) == 1
+        assert f.complex_inverse(16, x, y) == 0            # no inverse direction
+        assert i.complex_inverse(16, x, y) == 1
+        assert i.complex_forward(16, x, y) == 0
+        assert f.real_forward(16, x, y) == 1
+        assert f.real_inverse(16, x, y, t) == 0
+        assert i.real_inverse(16, x, y, t) == 1
+        assert i.real_inverse(16, x, y, None) == 0          # tmpBuf NULL
+        assert f.complex_forward(32, x, y) == 0             # n > nMax
+        assert f.complex_forward(12, x, y) == 0             # not a power of two
+        assert f.complex_forward(0, x, y) == 0
+        assert f.complex_forward(-2147483648, x, y) == 0    # INT_MIN is rejected here
+        assert f.complex_forward(16, x, x) == 0             # in == out
+        assert f.complex_forward(16, None, y) == 0
+        assert f.real_forward(16, x, None) == 0
+        assert f.complex_forward_batch(16, -1, x, y) == 0
+        assert f.complex_forward_batch(16, 0, x, y) == 1
+        assert f.complex_forward_batch(16, 2, x, y) == 1
+
+
+def test_caller_buffer_context(torch, C):
+    """Query mode, then a context placed in caller memory (src/ckfft/context.cpp:68-91)."""
+    size = ctypes.c_size_t(0)
+    assert not C.CkFftInit(1024, 3, None, ctypes.byref(size))
+    buf = ctypes.create_string_buffer(size.value)
+    h = C.CkFftInit(1024, 3, buf, ctypes.byref(size))
+    assert h == ctypes.addressof(buf)
+    x = make_input("cfwd", 1024, 1, 11)
+    xt = dev(torch, x)
+    y = torch.empty_like(xt)
+    assert C.CkFftComplexForwardBatch(h, 1024, 1, xt.data_ptr(), y.data_ptr(), None) == 1
+    torch.cuda.synchronize()
+    check("cfwd", 1024, y.cpu().numpy(), x)
+    C.CkFftShutdown(h)                 # does not free the caller's buffer
+    assert ctypes.sizeof(buf) == size.value
+
+
+# ------------------------------------------------------------------------- reference fixtures
+
+def test_golden_vectors_on_gpu(torch, ctx):
+    """Against the genuine reference's outputs on its own input (src/test/input.txt)."""
+    g = dict(np.load(os.path.join(GOLDEN, "golden_small.npz")))
+    inp = O.reference_input_txt()
+    for k in range(0, 13):
+        n = 1 << k
+        x = inp[: 2 * n][None]
+        for kind in ("cfwd", "cinv"):
+            got = run_batch(torch, ctx, kind, n, x)[0]
+            assert O.split_norm_err(got[None], g[f"{kind}_{n}"][None]) <= TOL_REF, (kind, n)
+        xr = np.ascontiguousarray(x[:, 0::2])
+        got = run_batch(torch, ctx, "rfwd", n, xr)[0]
+        assert O.split_norm_err(got[None], g[f"rfwd_{n}"][None]) <= TOL_REF, n
+        got = run_batch(torch, ctx, "rinv", n, g[f"rfwd_{n}"][None])[0]
+        assert O.split_norm_err(got[None], g[f"rinv_{n}"][None]) <= TOL_REF, n
+
+
+def test_reference_regression_checks_on_gpu(torch, ctx):
+    """src/test/test.cpp:754-864 on the GPU path: RMS vs an independent FFT <= 1e-3, and
+    real forward * 0.5 == complex forward of the same data with imag = 0."""
+    inp = O.reference_input_txt()
+    for k in range(0, 13):
+        n = 1 << k
+        x = inp[: 2 * n][None]
+        for kind in ("cfwd", "cinv"):
+            got = run_batch(torch, ctx, kind, n, x)[0]
+            want = fp64_out(kind, n, x)[0]
+            assert float(np.sqrt(np.mean((got - want) ** 2))) <= 1e-3
+        xr = np.ascontiguousarray(x[:, 0::2])
+        xc = np.zeros((1, 2 * n), np.float32)
+        xc[:, 0::2] = xr
+        c = run_batch(torch, ctx, "cfwd", n, xc)[0][: 2 * (n // 2 + 1)]
+        r = run_batch(torch, ctx, "rfwd", n, xr)[0] * 0.5
+        assert float(np.sqrt(np.mean((r - c) ** 2))) <= 1e-3
+
+
+def test_example_program_on_gpu(torch, C):
+    """src/example/main.cpp:44-85 through the drop-in calls with host buffers."""
+    ex = np.fromfile(os.path.join(GOLDEN, "example_in.f32"), dtype="<f4")
+    with C.Context(1024) as c:
+        y = np.zeros(2 * 513, np.float32)
+        assert c.real_forward(1024, ex, y) == 1
+        x = np.zeros(1024, np.float32)
+        assert c.real_inverse(1024, y, x, np.zeros(2 * 513, np.float32)) == 1
+    assert float(np.sum((x / 2048.0 - ex) ** 2)) < 1e-9
+
+
+# ------------------------------------------------------------------------- workload configs
+
+def test_config_complex_4096_x1024_round_trip(torch, ctx):
+    n, count = 4096, 1024
+    x = O.uniform_pm1(0xC0FFEE, count * 2 * n).reshape(count, 2 * n)
+    y = run_batch(torch, ctx, "cfwd", n, x)
+    check("cfwd", n, y, x)
+    z = run_batch(torch, ctx, "cinv", n, y)
+    check("cinv", n, z, y)
+    assert np.max(np.abs(z / n - x)) < 1e-5
+
+
+def test_config_real_65536_x256_round_trip(torch, ctx):
+    n, count = 65536, 256
+    x = O.normal01(0xBEEF, count * n).reshape(count, n)
+    y = run_batch(torch, ctx, "rfwd", n, x)
+    check("rfwd", n, y, x)
+    z = run_batch(torch, ctx, "rinv", n, y)
+    check("rinv", n, z, y)
+    assert np.max(np.abs(z / (2 * n) - x)) < 1e-4
