@@ -19,11 +19,22 @@ __all__ = [
     "LIB_PATH", "lib", "CkFftComplex", "kCkFftDirection_Forward", "kCkFftDirection_Inverse",
     "kCkFftDirection_Both", "CkFftInit", "CkFftShutdown", "CkFftComplexForward", "CkFftComplexInverse",
     "CkFftRealForward", "CkFftRealInverse", "CkFftComplexForwardBatch", "CkFftComplexInverseBatch",
-    "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "Context",
+    "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "CkFftGpuLastError", "Context",
     "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
+
+# One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7.  If this
+# library were loaded first it would bring in /opt/rocm's copy and torch would then
+# load a second runtime beside it, and the two do not share devices, streams or
+# allocations.  Loading torch first makes the dynamic linker bind this library to
+# torch's already-loaded runtime.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:  # a process without torch uses /opt/rocm's runtime only
+    _torch = None
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with `python ckfft_amd/build.py` "
                       "(there is no CPU fallback)")
@@ -55,6 +66,7 @@ _SIGS = {
     "CkFftRealInverseBatch": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     "CkFftGetTwiddles": (_i, [_vp, _i, _vp]),
     "CkFftGpuVersion": (ctypes.c_char_p, []),
+    "CkFftGpuLastError": (ctypes.c_char_p, []),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -95,7 +107,8 @@ class Context:
         self.direction = direction
         self.handle = CkFftInit(n_max, direction, None, None)
         if not self.handle:
-            raise RuntimeError(f"CkFftInit({n_max}, {direction}) failed (invalid size/direction or no GPU)")
+            raise RuntimeError(f"CkFftInit({n_max}, {direction}) failed: "
+                               f"{CkFftGpuLastError().decode()} (or invalid size/direction)")
 
     def close(self):
         if self.handle:

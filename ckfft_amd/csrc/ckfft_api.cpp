@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -74,7 +75,21 @@ bool is_device_pointer(const void* p)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-int fail(hipError_t) { return 0; }
+// Last failure on this thread, for CkFftGpuLastError().
+thread_local char g_last_error[256] = "no error";
+
+bool record(hipError_t e, const char* where)
+{
+    if (e == hipSuccess) return true;
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+    return false;
+}
+
+int fail(hipError_t e, const char* where)
+{
+    record(e, where);
+    return 0;
+}
 
 // One synchronous transform, as every drop-in call is.
 int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
@@ -85,8 +100,8 @@ int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
     hipStream_t s = nullptr;
     if (din && dout) {
         hipError_t e = launch_transform(kind, log2n, 1, in, out, c->d_pyramid, c->log_pyramid, s);
-        if (e != hipSuccess) return fail(e);
-        return hipStreamSynchronize(s) == hipSuccess ? 1 : 0;
+        if (e != hipSuccess) return fail(e, "launch");
+        return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
     }
     // Host buffers: stage through the context's device buffer.
     const size_t ib = in_bytes(kind, n), ob = out_bytes(kind, n);
@@ -97,16 +112,16 @@ int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
         if (c->d_staging) (void)hipFree(c->d_staging);
         c->d_staging = nullptr;
         c->staging_bytes = 0;
-        if (hipMalloc(&c->d_staging, need) != hipSuccess) return 0;
+        if (!record(hipMalloc(&c->d_staging, need), "hipMalloc(staging)")) return 0;
         c->staging_bytes = need;
     }
     char* st = (char*)c->d_staging;
     const void* src = din ? in : (const void*)st;
     void* dst = dout ? out : (void*)(st + ib_al);
-    if (!din && hipMemcpyAsync(st, in, ib, hipMemcpyHostToDevice, s) != hipSuccess) return 0;
-    if (launch_transform(kind, log2n, 1, src, dst, c->d_pyramid, c->log_pyramid, s) != hipSuccess) return 0;
-    if (!dout && hipMemcpyAsync(out, dst, ob, hipMemcpyDeviceToHost, s) != hipSuccess) return 0;
-    return hipStreamSynchronize(s) == hipSuccess ? 1 : 0;
+    if (!din && !record(hipMemcpyAsync(st, in, ib, hipMemcpyHostToDevice, s), "copy in")) return 0;
+    if (!record(launch_transform(kind, log2n, 1, src, dst, c->d_pyramid, c->log_pyramid, s), "launch")) return 0;
+    if (!dout && !record(hipMemcpyAsync(out, dst, ob, hipMemcpyDeviceToHost, s), "copy out")) return 0;
+    return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
 }
 
 int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void* out, void* stream)
@@ -119,7 +134,7 @@ int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void
     if (!is_device_pointer(in) || !is_device_pointer(out)) return 0;
     hipError_t e = launch_transform(kind, log2n, count, in, out, c->d_pyramid, c->log_pyramid,
                                     (hipStream_t)stream);
-    return e == hipSuccess ? 1 : 0;
+    return record(e, "launch") ? 1 : 0;
 }
 
 }  // namespace
@@ -141,14 +156,15 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     }
 
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) {
+    if (!record(hipGetDevice(&dev), "hipGetDevice")) {
         (void)hipGetLastError();
         return nullptr;
     }
     const int log_pyr = std::max(1, std::min(ilog2(nMax), kLogMaxPyramid));
     float2* pyr = nullptr;
-    if (hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)) != hipSuccess) return nullptr;
-    if (launch_twiddle_pyramid(pyr, log_pyr, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+    if (!record(hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)), "hipMalloc(twiddles)")) return nullptr;
+    if (!record(launch_twiddle_pyramid(pyr, log_pyr, nullptr), "twiddle launch") ||
+        !record(hipStreamSynchronize(nullptr), "twiddle synchronize")) {
         (void)hipFree(pyr);
         return nullptr;
     }
@@ -260,10 +276,13 @@ int CkFftGetTwiddles(CkFftContext* context, int L, CkFftComplex* out)
 {
     if (!context || context->magic != kContextMagic || !out) return 0;
     if (!pow2_positive(L) || ilog2(L) > context->log_pyramid) return 0;
-    return hipMemcpy(out, context->d_pyramid + L, sizeof(float2) * (size_t)L, hipMemcpyDeviceToHost) == hipSuccess
+    return record(hipMemcpy(out, context->d_pyramid + L, sizeof(float2) * (size_t)L, hipMemcpyDeviceToHost),
+                  "copy twiddles")
                ? 1
                : 0;
 }
+
+const char* CkFftGpuLastError(void) { return g_last_error; }
 
 const char* CkFftGpuVersion(void)
 {

@@ -40,6 +40,10 @@ int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
  * L must be a power of two no larger than the context's table.  Returns 1/0. */
 int CkFftGetTwiddles(CkFftContext* context, int L, CkFftComplex* out);
 
+/* Description of the last GPU-side failure on the calling thread (a HIP error and
+ * the step that returned it), or "no error".  Valid until the thread's next call. */
+const char* CkFftGpuLastError(void);
+
 /* Library version and build information as a static string. */
 const char* CkFftGpuVersion(void);
 

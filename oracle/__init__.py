@@ -200,10 +200,20 @@ def _to_c128(x: np.ndarray) -> np.ndarray:
 
 
 def exact_twiddle(k: np.ndarray, n: int, sign: int = -1) -> np.ndarray:
-    """e^{sign * 2 pi i k / n} in float64 with the angle reduced exactly (k mod n)."""
+    """e^{sign * 2 pi i k / n} in float64 with the angle reduced exactly.
+
+    x = 2k/n is split into q quarter turns and a remainder d in [-1/4, 1/4] (both exact
+    for a power-of-two n), sin/cos are taken of pi*d only, and the quarter turns are
+    applied by swapping/negating -- so multiples of a quarter turn come out exact."""
     k = np.asarray(k, dtype=np.int64) % n
-    frac = k.astype(np.float64) / n          # exact: n is a power of two
-    return np.cos(2 * np.pi * frac) + sign * 1j * np.sin(2 * np.pi * frac)
+    x = 2.0 * k.astype(np.float64) / n       # exact: n is a power of two
+    q = np.rint(2.0 * x)
+    d = x - 0.5 * q
+    s, c = np.sin(np.pi * d), np.cos(np.pi * d)
+    qi = q.astype(np.int64) & 3
+    sn = np.select([qi == 0, qi == 1, qi == 2], [s, c, -s], -c)
+    cs = np.select([qi == 0, qi == 1, qi == 2], [c, -s, -c], s)
+    return cs + sign * 1j * sn
 
 
 def fp64_complex(x, inverse: bool) -> np.ndarray:

@@ -31,13 +31,13 @@ def torch():
 
 
 @pytest.fixture(scope="module")
-def C():
+def C(torch):
     import ckfft_amd
     return ckfft_amd
 
 
 @pytest.fixture(scope="module")
-def ctx(C, torch):
+def ctx(torch, C):
     c = C.Context(1 << 16, C.kCkFftDirection_Both)
     yield c
     c.close()
