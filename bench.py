@@ -124,6 +124,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-signal leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
+                         "HBM counters FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the transform path "
@@ -137,6 +140,16 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        nbytes = 128 << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read_per_launch": nbytes,
+                          "bytes_written_per_launch": nbytes, "launches": 5}))
+        return
     n, B = N_REAL, BATCH
     half = n // 2 + 1
 
