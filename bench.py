@@ -199,7 +199,7 @@ def main():
     workload = f"real_fft_{n}_x{B}"
     traffic = load_traffic(workload)
     roofline = {
-        "bound": "hbm", "kernel": f"fft_lds_kernel<15,5,1024,{dom_name}>",
+        "bound": "hbm", "kernel": f"fft_lds_kernel ({dom_name.replace('_', ' ')}, n={n}: one workgroup per signal)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": (traffic or {}).get(dom_name),

@@ -91,6 +91,11 @@ int fail(hipError_t e, const char* where)
     return 0;
 }
 
+DeviceTables tables(const CkFftContext* c)
+{
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+}
+
 // One synchronous transform, as every drop-in call is.
 int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
 {
@@ -99,7 +104,7 @@ int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
     const bool din = is_device_pointer(in), dout = is_device_pointer(out);
     hipStream_t s = nullptr;
     if (din && dout) {
-        hipError_t e = launch_transform(kind, log2n, 1, in, out, c->d_pyramid, c->log_pyramid, s);
+        hipError_t e = launch_transform(kind, log2n, 1, in, out, tables(c), s);
         if (e != hipSuccess) return fail(e, "launch");
         return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
     }
@@ -119,7 +124,7 @@ int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
     const void* src = din ? in : (const void*)st;
     void* dst = dout ? out : (void*)(st + ib_al);
     if (!din && !record(hipMemcpyAsync(st, in, ib, hipMemcpyHostToDevice, s), "copy in")) return 0;
-    if (!record(launch_transform(kind, log2n, 1, src, dst, c->d_pyramid, c->log_pyramid, s), "launch")) return 0;
+    if (!record(launch_transform(kind, log2n, 1, src, dst, tables(c), s), "launch")) return 0;
     if (!dout && !record(hipMemcpyAsync(out, dst, ob, hipMemcpyDeviceToHost, s), "copy out")) return 0;
     return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
 }
@@ -132,8 +137,7 @@ int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void
     if (log2n > max_log2n(kind)) return 0;
     if (count == 0) return 1;
     if (!is_device_pointer(in) || !is_device_pointer(out)) return 0;
-    hipError_t e = launch_transform(kind, log2n, count, in, out, c->d_pyramid, c->log_pyramid,
-                                    (hipStream_t)stream);
+    hipError_t e = launch_transform(kind, log2n, count, in, out, tables(c), (hipStream_t)stream);
     return record(e, "launch") ? 1 : 0;
 }
 
@@ -163,20 +167,32 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     const int log_pyr = std::max(1, std::min(ilog2(nMax), kLogMaxPyramid));
     float2* pyr = nullptr;
     if (!record(hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)), "hipMalloc(twiddles)")) return nullptr;
+    const int log_max = std::min(ilog2(nMax), kLogMaxSupported);
+    float2* fine = nullptr;
+    if (log_max > kLogMaxPyramid &&
+        !record(hipMalloc(&fine, sizeof(float2) * ((size_t)(log_max - kLogMaxPyramid) << kLogFine)),
+                "hipMalloc(fine twiddles)")) {
+        (void)hipFree(pyr);
+        return nullptr;
+    }
     if (!record(launch_twiddle_pyramid(pyr, log_pyr, nullptr), "twiddle launch") ||
+        !record(launch_twiddle_fine(fine, log_max, nullptr), "fine twiddle launch") ||
         !record(hipStreamSynchronize(nullptr), "twiddle synchronize")) {
         (void)hipFree(pyr);
+        if (fine) (void)hipFree(fine);
         return nullptr;
     }
 
     void* mem = buf ? buf : std::malloc(need);
     if (!mem) {
         (void)hipFree(pyr);
+        if (fine) (void)hipFree(fine);
         return nullptr;
     }
     std::mutex* mtx = new (std::nothrow) std::mutex();
     if (!mtx) {
         (void)hipFree(pyr);
+        if (fine) (void)hipFree(fine);
         if (!buf) std::free(mem);
         return nullptr;
     }
@@ -188,6 +204,8 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->device = dev;
     c->log_pyramid = log_pyr;
     c->d_pyramid = pyr;
+    c->d_fine = fine;
+    c->log_max = log_max;
     c->staging_mutex = mtx;
     c->d_staging = nullptr;
     c->staging_bytes = 0;
@@ -236,6 +254,7 @@ void CkFftShutdown(CkFftContext* context)
 {
     if (!context || context->magic != kContextMagic) return;
     (void)hipFree(context->d_pyramid);
+    if (context->d_fine) (void)hipFree(context->d_fine);
     if (context->d_staging) (void)hipFree(context->d_staging);
     delete context->staging_mutex;
     const int own = context->own_host;

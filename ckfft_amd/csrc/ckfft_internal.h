@@ -23,21 +23,44 @@ enum Kind : int {
 constexpr int kLogMaxLds = 15;
 // Largest level of the twiddle pyramid kept on the device.
 constexpr int kLogMaxPyramid = 16;
+// Fine twiddle tables for levels above the pyramid hold 2^kLogFine entries each.
+constexpr int kLogFine = 12;
+// Largest supported transform (complex points or real samples): 2^28.  Twiddles of a
+// level L > 2^16 are a coarse pyramid entry (level L / 2^12 <= 2^16) times a fine one.
+constexpr int kLogMaxSupported = kLogMaxPyramid + kLogFine;
+// Radix range of one multi-pass stage.
+constexpr int kLogMinPass = 6;
+constexpr int kLogMaxPass = 12;
 
-// Twiddle pyramid layout: level L (a power of two) occupies entries [L, 2L) and
-// entry L + k holds exp(-2 pi i k / L) in FP32, rounded from an FP64 evaluation.
-// Entry 0 is unused (set to 1).
+// Device twiddle tables of a context.
+//  * pyramid: level L (a power of two <= 2^log_pyramid) occupies entries [L, 2L);
+//    entry L + k holds exp(-2 pi i k / L).  Entry 0 is unused (set to 1).
+//  * fine: for each level L = 2^l, l in (16, log_max], 2^12 entries exp(-2 pi i m / L),
+//    m < 2^12, at offset (l - 17) * 2^12.
+// Every entry is evaluated in FP64 with an exact argument reduction and rounded once.
+struct DeviceTables {
+    const float2* pyramid;
+    int log_pyramid;
+    const float2* fine;      // null when log_max <= 16
+    int log_max;             // largest log2(n) the tables serve
+};
 
-// Launchers (return hipSuccess or an error; never allocate).
+// Launchers (return hipSuccess or an error).
 hipError_t launch_twiddle_pyramid(float2* table, int log_lmax, hipStream_t stream);
+hipError_t launch_twiddle_fine(float2* table, int log_max, hipStream_t stream);
 
 // Batched transform of `count` signals of size n (n >= 1, power of two, n <= the
-// kind's limit).  Pointers are device memory with the layouts of ckfft_gpu.h.
+// kind's limit).  Pointers are device memory with the layouts of ckfft_gpu.h.  Sizes
+// above one workgroup take a stream-ordered scratch allocation for the intermediate.
 hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, void* out,
-                            const float2* pyramid, int log_pyramid, hipStream_t stream);
+                            const DeviceTables& tables, hipStream_t stream);
 
 // Largest log2(n) launch_transform supports for a kind.
 int max_log2n(Kind kind);
+
+// Multi-pass path (fft_pass.hip) for sizes above one workgroup.
+hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, void* out,
+                            const DeviceTables& tables, hipStream_t stream);
 
 }  // namespace ckfft_gpu
 
@@ -50,6 +73,8 @@ struct _CkFftContext {
     int device;              // HIP device the tables live on
     int log_pyramid;         // levels 1..2^log_pyramid are on the device
     float2* d_pyramid;       // device twiddle pyramid, 2 << log_pyramid entries
+    float2* d_fine;          // fine twiddle tables for levels above 2^16 (or null)
+    int log_max;             // largest log2(n) the tables serve
     // Staging for host-pointer calls (grown on demand, guarded by staging_mutex).
     std::mutex* staging_mutex;
     void* d_staging;

@@ -18,18 +18,9 @@ hipError_t launch_lds(Kind kind, int logc, int count, const void* in, void* out,
 //   x = 2k/L (exact, L a power of two), n = round(2x), d = x - n/2 in [-1/4, 1/4]
 //   (exact), sincospi(d) in FP64, then a rotation by n quarter turns (exact).
 // Quarter-turn entries such as L/4 come out exactly (0, -1).
-__global__ void twiddle_pyramid_kernel(float2* __restrict__ table, int log_lmax)
+__device__ float2 twiddle_fp64(long long k, int lg)
 {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = 2ll << log_lmax;
-    if (e >= total) return;
-    if (e == 0) {
-        table[0] = make_float2(1.f, 0.f);
-        return;
-    }
-    const int lg = 63 - __builtin_clzll((unsigned long long)e);   // level L = 2^lg
-    const long long k = e - (1ll << lg);
-    const double x = ldexp((double)k, 1 - lg);                   // 2k/L in [0, 2)
+    const double x = ldexp((double)k, 1 - lg);                   // 2k/L in [0, 2), L = 2^lg
     const double nq = rint(2.0 * x);                             // quarter turns
     const double d = x - 0.5 * nq;                               // |d| <= 1/4, exact
     double s, c;
@@ -41,7 +32,39 @@ __global__ void twiddle_pyramid_kernel(float2* __restrict__ table, int log_lmax)
     case 2: sn = -s; cs = -c; break;
     default: sn = -c; cs = s; break;
     }
-    table[e] = make_float2((float)cs, (float)(-sn));
+    return make_float2((float)cs, (float)(-sn));
+}
+
+__global__ void twiddle_pyramid_kernel(float2* __restrict__ table, int log_lmax)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = 2ll << log_lmax;
+    if (e >= total) return;
+    if (e == 0) {
+        table[0] = make_float2(1.f, 0.f);
+        return;
+    }
+    const int lg = 63 - __builtin_clzll((unsigned long long)e);   // level L = 2^lg
+    table[e] = twiddle_fp64(e - (1ll << lg), lg);
+}
+
+// Fine tables: level 2^l for l in (16, log_max], entries m < 2^12.
+__global__ void twiddle_fine_kernel(float2* __restrict__ table, int log_max)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)(log_max - kLogMaxPyramid) << kLogFine;
+    if (e >= total) return;
+    const int lg = kLogMaxPyramid + 1 + (int)(e >> kLogFine);
+    table[e] = twiddle_fp64(e & ((1 << kLogFine) - 1), lg);
+}
+
+hipError_t launch_twiddle_fine(float2* table, int log_max, hipStream_t stream)
+{
+    if (log_max <= kLogMaxPyramid) return hipSuccess;
+    const long long total = (long long)(log_max - kLogMaxPyramid) << kLogFine;
+    hipLaunchKernelGGL(twiddle_fine_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, table,
+                       log_max);
+    return hipGetLastError();
 }
 
 hipError_t launch_twiddle_pyramid(float2* table, int log_lmax, hipStream_t stream)
@@ -141,23 +164,23 @@ static hipError_t launch_tiny(Kind kind, int log2n, int count, const void* in, v
 
 // ---- dispatch ---------------------------------------------------------------------------
 
-int max_log2n(Kind kind)
-{
-    return (kind == kRealForward || kind == kRealInverse) ? kLogMaxLds + 1 : kLogMaxLds;
-}
+int max_log2n(Kind) { return kLogMaxSupported; }
 
-hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, void* out, const float2* pyramid,
-                            int log_pyramid, hipStream_t stream)
+hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, void* out, const DeviceTables& tb,
+                            hipStream_t stream)
 {
-    if (log2n < 0 || log2n > max_log2n(kind) || count < 0) return hipErrorInvalidValue;
+    if (log2n < 0 || log2n > max_log2n(kind) || log2n > tb.log_max || count < 0) return hipErrorInvalidValue;
     if (count == 0) return hipSuccess;
     const bool real = (kind == kRealForward || kind == kRealInverse);
     if (log2n <= (real ? 2 : 1)) return launch_tiny(kind, log2n, count, in, out, stream);
     // complex size 2^logc; the real kinds run a half-size complex transform and read
-    // the pyramid at level 2^log2n for their pack/unpack twiddles
+    // the twiddles of level 2^log2n for their pack/unpack
     const int logc = real ? log2n - 1 : log2n;
-    if (log_pyramid < log2n) return hipErrorInvalidValue;
-    return launch_lds(kind, logc, count, in, out, pyramid, stream);
+    if (logc <= kLogMaxLds) {
+        if (tb.log_pyramid < log2n) return hipErrorInvalidValue;
+        return launch_lds(kind, logc, count, in, out, tb.pyramid, stream);
+    }
+    return launch_multipass(kind, log2n, count, in, out, tb, stream);
 }
 
 }  // namespace ckfft_gpu

@@ -1,0 +1,194 @@
+// fft_stockham.h -- register/LDS Stockham building blocks shared by the one-workgroup
+// kernels (fft_lds.hip) and the multi-pass kernels (fft_pass.hip).
+//
+// A transform of N points is owned by T = N/P threads, each keeping P points in
+// registers.  Stage s has radix R (16, except one smaller "odd" stage) and span Ns:
+// butterfly j reads point j + r*N/R and writes (j/Ns)*Ns*R + j%Ns + r*Ns (Stockham
+// autosort), so input and output are in natural order.  Between stages the data moves
+// through LDS as separate real/imaginary planes padded by one word per 16.
+#pragma once
+#include "ckfft_internal.h"
+#include "fft_device.h"
+
+namespace ckfft_gpu {
+
+// Padded LDS index: one unused word after every 16.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+constexpr int lds_span(int n) { return n + (n >> 4); }
+
+// Radix plan: every stage is radix 2^RMAX_LOG except one smaller "odd" stage, placed
+// first (SMALL_LAST = false) or last (SMALL_LAST = true).
+template <int LOGN, int LOGP, bool SMALL_LAST = false>
+struct Plan {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int P = 1 << LOGP;          // points per thread
+    static constexpr int T = N / P;              // threads per transform
+    static constexpr int RMAX_LOG = LOGP < 4 ? LOGP : 4;
+    static constexpr int NSTAGES = (LOGN + RMAX_LOG - 1) / RMAX_LOG;
+    static constexpr int SMALL_LOG = LOGN - (NSTAGES - 1) * RMAX_LOG;
+    static constexpr int rlog(int s)
+    {
+        return (SMALL_LAST ? s == NSTAGES - 1 : s == 0) ? SMALL_LOG : RMAX_LOG;
+    }
+    static constexpr int nslog(int s)
+    {
+        int a = 0;
+        for (int i = 0; i < s; ++i) a += rlog(i);
+        return a;
+    }
+    static_assert(LOGP <= LOGN, "more points per thread than points");
+    static_assert(SMALL_LOG >= 1 && SMALL_LOG <= RMAX_LOG, "bad radix plan");
+};
+
+// Butterflies per thread in stage S.
+template <class PL, int S>
+constexpr int bflies() { return PL::P >> PL::rlog(S); }
+
+// Thread -> butterfly assignment of stage S.  Normally thread t owns butterflies
+// t + b*T.  In a "mirror-closed" stage (MIRROR, needs an even count B >= 2) thread t owns
+// j = t + b*T for b < B/2 and M - j for the other half (M = N/R butterflies; thread 0
+// takes M/2 in place of M - 0), so that the points k and N - k of every pair live in the
+// same thread: element (j, r) is point j + r*M and its mirror N - (j + r*M) is element
+// (M - j, R-1-r).  The real pack/unpack then needs no exchange.
+template <class PL, int S, bool MIRROR>
+__device__ __forceinline__ int bfly(int t, int b)
+{
+    constexpr int B = bflies<PL, S>();
+    constexpr int M = PL::N >> PL::rlog(S);
+    if constexpr (!MIRROR) {
+        return t + b * PL::T;
+    } else {
+        static_assert(B >= 2 && (B % 2) == 0, "mirror-closed stage needs an even butterfly count");
+        if (b < B / 2) return t + b * PL::T;
+        const int jl = t + (b - B / 2) * PL::T;
+        return jl == 0 ? M / 2 : M - jl;
+    }
+}
+
+// ---- stage arithmetic --------------------------------------------------------
+
+// Multiply a[1..R-1] by w^r, w = lvl[k].  Only the power-of-two powers w, w^2, w^4, w^8
+// are read from the table (exact FP64-rounded values); the others are products of at
+// most four of them, which keeps the twiddle working set per butterfly small.
+template <int R, bool INV>
+__device__ __forceinline__ void twiddle_mul(float2* a, const float2* __restrict__ lvl, int k)
+{
+    if constexpr (R >= 2) {
+        const float2 w1 = dirw<INV>(lvl[k]);
+        if constexpr (R == 2) {
+            a[1] = cmul(a[1], w1);
+        } else {
+            const float2 w2 = dirw<INV>(lvl[2 * k]);
+            const float2 w3 = cmul(w1, w2);
+            a[1] = cmul(a[1], w1);
+            a[2] = cmul(a[2], w2);
+            a[3] = cmul(a[3], w3);
+            if constexpr (R >= 8) {
+                const float2 w4 = dirw<INV>(lvl[4 * k]);
+                a[4] = cmul(a[4], w4);
+                a[5] = cmul(a[5], cmul(w1, w4));
+                a[6] = cmul(a[6], cmul(w2, w4));
+                const float2 w7 = cmul(w3, w4);
+                a[7] = cmul(a[7], w7);
+                if constexpr (R == 16) {
+                    const float2 w8 = dirw<INV>(lvl[8 * k]);
+                    a[8] = cmul(a[8], w8);
+                    a[9] = cmul(a[9], cmul(w1, w8));
+                    a[10] = cmul(a[10], cmul(w2, w8));
+                    a[11] = cmul(a[11], cmul(w3, w8));
+                    const float2 w12 = cmul(w4, w8);
+                    a[12] = cmul(a[12], w12);
+                    a[13] = cmul(a[13], cmul(w1, w12));
+                    a[14] = cmul(a[14], cmul(w2, w12));
+                    a[15] = cmul(a[15], cmul(w3, w12));
+                }
+            }
+        }
+    }
+}
+
+template <class PL, int S, bool INV, bool MIRROR = false>
+__device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw)
+{
+    constexpr int R = 1 << PL::rlog(S);
+    constexpr int NS = 1 << PL::nslog(S);
+    constexpr int B = PL::P / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        if constexpr (NS > 1) {
+            const int k = bfly<PL, S, MIRROR>(t, b) & (NS - 1);
+            twiddle_mul<R, INV>(v + b * R, tw + NS * R, k);     // level NS*R of the pyramid
+        }
+        dft<R, INV>(v + b * R);
+    }
+}
+
+// Index of element r of butterfly j at the input of stage S.
+template <class PL, int S>
+__device__ __forceinline__ int in_index(int j, int r)
+{
+    return j + r * (PL::N >> PL::rlog(S));
+}
+
+// Index of element r of butterfly j at the output of stage S.
+template <class PL, int S>
+__device__ __forceinline__ int out_index(int j, int r)
+{
+    constexpr int R = 1 << PL::rlog(S);
+    constexpr int NS = 1 << PL::nslog(S);
+    return (j / NS) * NS * R + (j & (NS - 1)) + r * NS;
+}
+
+// COMP: 1 = real plane, 2 = imaginary plane, 3 = both.
+template <class PL, int S, int COMP, bool MIRROR = false>
+__device__ __forceinline__ void lds_write(const float2* v, int t, float* re, float* im)
+{
+    constexpr int R = 1 << PL::rlog(S);
+    constexpr int B = PL::P / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lds_pad(out_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+            if constexpr (COMP & 1) re[i] = v[b * R + r].x;
+            if constexpr (COMP & 2) im[i] = v[b * R + r].y;
+        }
+}
+
+template <class PL, int S, int COMP, bool MIRROR = false>
+__device__ __forceinline__ void lds_read(float2* v, int t, const float* re, const float* im)
+{
+    constexpr int R = 1 << PL::rlog(S);
+    constexpr int B = PL::P / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lds_pad(in_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+            if constexpr (COMP & 1) v[b * R + r].x = re[i];
+            if constexpr (COMP & 2) v[b * R + r].y = im[i];
+        }
+}
+
+// Move data from the output layout of stage SW to the input layout of stage SR.
+template <class PL, int SW, int SR, bool SPLIT, bool MW, bool MR>
+__device__ __forceinline__ void exchange(float2* v, int t, float* re, float* im)
+{
+    if constexpr (!SPLIT) {
+        lds_write<PL, SW, 3, MW>(v, t, re, im);
+        __syncthreads();
+        lds_read<PL, SR, 3, MR>(v, t, re, im);
+        __syncthreads();
+    } else {
+        lds_write<PL, SW, 1, MW>(v, t, re, im);
+        __syncthreads();
+        lds_read<PL, SR, 1, MR>(v, t, re, im);
+        __syncthreads();
+        lds_write<PL, SW, 2, MW>(v, t, re, im);
+        __syncthreads();
+        lds_read<PL, SR, 2, MR>(v, t, re, im);
+        __syncthreads();
+    }
+}
+
+}  // namespace ckfft_gpu

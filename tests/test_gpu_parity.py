@@ -326,3 +326,61 @@ def test_config_real_65536_x256_round_trip(torch, ctx):
     z = run_batch(torch, ctx, "rinv", n, y)
     check("rinv", n, z, y)
     assert np.max(np.abs(z / (2 * n) - x)) < 1e-4
+
+
+# ------------------------------------------------------------------------- multi-pass sizes
+
+@pytest.fixture(scope="module")
+def ctx_big(torch, C):
+    c = C.Context(1 << 25, C.kCkFftDirection_Both)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("kind", ["cfwd", "cinv"])
+@pytest.mark.parametrize("k", range(16, 26))
+def test_complex_multipass_sizes(torch, ctx_big, kind, k):
+    """2 passes for 2^16..2^24 (every radix split 2^8 x 2^8 .. 2^12 x 2^12), 3 above."""
+    n = 1 << k
+    count = 2 if k <= 20 else 1
+    x = make_input(kind, n, count, 8000 + 10 * k)
+    check(kind, n, run_batch(torch, ctx_big, kind, n, x), x)
+
+
+@pytest.mark.parametrize("kind", ["rfwd", "rinv"])
+@pytest.mark.parametrize("k", [17, 18, 20, 22])
+def test_real_multipass_sizes(torch, ctx_big, kind, k):
+    n = 1 << k
+    count = 3 if k <= 18 else 1
+    x = make_input(kind, n, count, 9000 + k)
+    check(kind, n, run_batch(torch, ctx_big, kind, n, x), x)
+
+
+def test_multipass_plain_equals_batch(torch, ctx_big):
+    n = 1 << 17
+    x = make_input("cfwd", n, 3, 9100)
+    batch = run_batch(torch, ctx_big, "cfwd", n, x)
+    xt = dev(torch, x)
+    y = torch.empty(2 * n, dtype=torch.float32, device="cuda")
+    assert ctx_big.complex_forward(n, xt[1], y) == 1
+    assert np.array_equal(y.cpu().numpy().view(np.uint32), batch[1].view(np.uint32))
+
+
+def test_multipass_max_count_independent(torch, C):
+    n = 1 << 18
+    x = make_input("rfwd", n, 1, 9200)
+    outs = []
+    for m in (n, 1 << 22):
+        with C.Context(m) as c:
+            outs.append(run_batch(torch, c, "rfwd", n, x))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_config_single_complex_2_24(torch, ctx_big):
+    """One complex transform of 2^24 points (128 MiB in, 128 MiB out), forward and inverse."""
+    n = 1 << 24
+    x = O.uniform_pm1(0x2424, 2 * n)[None]
+    y = run_batch(torch, ctx_big, "cfwd", n, x)
+    check("cfwd", n, y, x)
+    z = run_batch(torch, ctx_big, "cinv", n, y)
+    assert np.max(np.abs(z / n - x)) < 1e-5
