@@ -39,10 +39,8 @@ template <class PL, int KIND>
 constexpr int mirror_stage()
 {
     if constexpr (KIND == kRealInverse) {
-        // at 2^15 the paired unpack raises register pressure into spills and measured
-        // slower than reading each mirror point again (a cache hit), so it is used below
         constexpr int B = bflies<PL, 0>();
-        return (B >= 2 && B % 2 == 0 && PL::N < (1 << 15)) ? 0 : -1;
+        return (B >= 2 && B % 2 == 0) ? 0 : -1;
     } else if constexpr (KIND == kRealForward) {
         constexpr int B = bflies<PL, PL::NSTAGES - 1>();
         return (B >= 2 && B % 2 == 0) ? PL::NSTAGES - 1 : -1;
