@@ -19,7 +19,9 @@ __all__ = [
     "LIB_PATH", "lib", "CkFftComplex", "kCkFftDirection_Forward", "kCkFftDirection_Inverse",
     "kCkFftDirection_Both", "CkFftInit", "CkFftShutdown", "CkFftComplexForward", "CkFftComplexInverse",
     "CkFftRealForward", "CkFftRealInverse", "CkFftComplexForwardBatch", "CkFftComplexInverseBatch",
-    "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "CkFftGpuLastError", "Context",
+    "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "CkFftGpuLastError", "Context", "BatchPlan", "CkFftBatchDesc",
+    "CkFftBatchInfo", "CkFftBatchPlanCreate", "CkFftBatchPlanExecute", "CkFftBatchPlanGetInfo",
+    "CkFftBatchPlanCheck", "CkFftBatchPlanDestroy", "CkFftBatchLastError", "CkFftAutotune", "CkFftLoadTuning",
     "EXPORTED_SYMBOLS",
 ]
 
@@ -50,6 +52,24 @@ kCkFftDirection_Forward = 1
 kCkFftDirection_Inverse = 2
 kCkFftDirection_Both = 3
 
+kCkFftKind_ComplexForward = 0
+kCkFftKind_ComplexInverse = 1
+kCkFftKind_RealForward = 2
+kCkFftKind_RealInverse = 3
+kCkFftBatch_Graph = 1
+
+
+class CkFftBatchDesc(ctypes.Structure):
+    """One transform of an uneven batch (include/ckfft/ckfft_gpu.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("input", ctypes.c_void_p),
+                ("output", ctypes.c_void_p)]
+
+
+class CkFftBatchInfo(ctypes.Structure):
+    _fields_ = [("transforms", ctypes.c_int), ("persistent_transforms", ctypes.c_int),
+                ("work_items", ctypes.c_int), ("workgroups", ctypes.c_int), ("extra_launches", ctypes.c_int),
+                ("algorithmic_bytes", ctypes.c_longlong)]
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 
@@ -67,6 +87,14 @@ _SIGS = {
     "CkFftGetTwiddles": (_i, [_vp, _i, _vp]),
     "CkFftGpuVersion": (ctypes.c_char_p, []),
     "CkFftGpuLastError": (ctypes.c_char_p, []),
+    "CkFftBatchPlanCreate": (_vp, [_vp, ctypes.POINTER(CkFftBatchDesc), _i, _i]),
+    "CkFftBatchPlanExecute": (_i, [_vp, _vp]),
+    "CkFftBatchPlanGetInfo": (_i, [_vp, ctypes.POINTER(CkFftBatchInfo)]),
+    "CkFftBatchPlanCheck": (_i, [_vp]),
+    "CkFftBatchPlanDestroy": (None, [_vp]),
+    "CkFftBatchLastError": (ctypes.c_char_p, []),
+    "CkFftAutotune": (_i, [_vp, ctypes.c_char_p]),
+    "CkFftLoadTuning": (_i, [_vp, ctypes.c_char_p]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -155,3 +183,47 @@ class Context:
 
     def twiddles(self, L: int, out) -> int:
         return CkFftGetTwiddles(self.handle, L, _ptr(out))
+
+    def autotune(self, cache_path=None) -> int:
+        return CkFftAutotune(self.handle, cache_path.encode() if cache_path else None)
+
+    def load_tuning(self, cache_path) -> int:
+        return CkFftLoadTuning(self.handle, cache_path.encode())
+
+
+class BatchPlan:
+    """An uneven batch run as one persistent launch (CkFftBatchPlan*).
+
+    ``transforms`` is a sequence of (kind, n, input, output) with device buffers.
+    """
+
+    def __init__(self, ctx: Context, transforms, graph: bool = False):
+        arr = (CkFftBatchDesc * max(1, len(transforms)))()
+        for i, (kind, n, x, y) in enumerate(transforms):
+            arr[i] = CkFftBatchDesc(kind, n, _ptr(x), _ptr(y))
+        self.ctx = ctx                         # keeps the context alive
+        self.handle = CkFftBatchPlanCreate(ctx.handle, arr, len(transforms), kCkFftBatch_Graph if graph else 0)
+        if not self.handle:
+            raise ValueError(f"CkFftBatchPlanCreate failed: {CkFftBatchLastError().decode()}")
+
+    def execute(self, stream=None) -> int:
+        return CkFftBatchPlanExecute(self.handle, _stream(stream))
+
+    def check(self) -> int:
+        return CkFftBatchPlanCheck(self.handle)
+
+    def info(self) -> dict:
+        inf = CkFftBatchInfo()
+        assert CkFftBatchPlanGetInfo(self.handle, ctypes.byref(inf)) == 1
+        return {f: getattr(inf, f) for f, _ in CkFftBatchInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            CkFftBatchPlanDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -3,7 +3,9 @@
 ``python ckfft_amd/build.py`` (run by path: the package itself refuses to import
 without the library this builds).  Every HIP source is
 compiled for gfx950 only, with packed-FP32 instruction selection turned off
-(``-Xclang -target-feature -Xclang -packed-fp32-ops``), then linked into one shared
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``) and no implicit multiply-add
+contraction (``-ffp-contract=off``: the kernels spell out their fused operations, so
+every kernel computing a transform rounds identically), then linked into one shared
 library next to this file.  Objects go to ``_build/`` at the repository root.
 """
 from __future__ import annotations
@@ -20,11 +22,11 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libckfft_amd.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["fft_lds.hip", "fft_pass.hip", "fft_misc.hip", "ckfft_api.cpp"]
+SOURCES = ["fft_lds.hip", "fft_pass.hip", "fft_batch.hip", "fft_misc.hip", "ckfft_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = [
-    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,

@@ -206,6 +206,7 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->d_pyramid = pyr;
     c->d_fine = fine;
     c->log_max = log_max;
+    std::memset(c->tune, 0, sizeof(c->tune));
     c->staging_mutex = mtx;
     c->d_staging = nullptr;
     c->staging_bytes = 0;

@@ -75,6 +75,9 @@ struct _CkFftContext {
     float2* d_pyramid;       // device twiddle pyramid, 2 << log_pyramid entries
     float2* d_fine;          // fine twiddle tables for levels above 2^16 (or null)
     int log_max;             // largest log2(n) the tables serve
+    // Tuned variant (points per thread 16 or 32) per [kind][log2 complex size], used by
+    // batch plans; every variant has the same arithmetic.
+    unsigned char tune[4][16];
     // Staging for host-pointer calls (grown on demand, guarded by staging_mutex).
     std::mutex* staging_mutex;
     void* d_staging;

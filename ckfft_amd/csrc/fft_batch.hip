@@ -1,0 +1,589 @@
+// fft_batch.hip -- uneven batches in one persistent launch, and the variant autotuner.
+//
+// The reference runs one transform per call on the calling thread
+// (src/ckfft/ckfft.cpp:36-114); it has no batching.  Here a batch of transforms of any
+// mix of sizes, kinds and directions (each with its own device buffers) runs as ONE
+// launch of a persistent kernel:
+//  * the host groups the transforms into work items -- a group of same-class transforms
+//    that fills one 512-thread workgroup, or one tile of a multi-pass transform -- and
+//    orders them largest-first (first-pass tiles first, so that no dequeued tile can
+//    wait on work nobody has taken);
+//  * one workgroup per CU loops: take the next item from an atomic queue, run it with
+//    the same transform bodies the batch kernels use (fft_lds.h, fft_pass.h);
+//  * a second-pass tile waits (bounded) until the transform's first-pass tiles have
+//    published their results (agent-scope release/acquire), so a complex 2^16 transform
+//    splits into 16 independent tiles;
+//  * the last workgroup to leave resets the queue and counters, so a plan re-executes
+//    (eagerly or as a replayed hipGraph) with no other launch.
+// Transforms the persistent kernel does not take (closed-form sizes, sizes above 2^16)
+// run through the ordinary launches after it, on the same stream.
+//
+// Autotuning: every class (kind, size) may run with 16 or 32 points per thread, which
+// changes waves per transform, transforms per workgroup and LDS tile, but never the
+// arithmetic (the radix plan and every twiddle are the same), so a tuned plan gives
+// bit-identical results to the single-signal calls.  CkFftAutotune times both variants
+// per class and keeps the faster in the context and, optionally, in a JSON cache.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ckfft/ckfft_gpu.h"
+#include "fft_lds.h"
+#include "fft_pass.h"
+
+namespace ckfft_gpu {
+
+struct BatchItem {
+    int cls;        // class id (lds_cls / kClsPass + ...)
+    int first;      // first transform (index into the BatchTransform table)
+    int count;      // transforms in this item (single-workgroup classes)
+    int tile;       // tile index (pass classes)
+};
+
+struct BatchTransform {
+    const float2* in;
+    float2* out;
+    float2* scratch;   // intermediate of a multi-pass transform
+    int dep;           // its dependency counter
+    int pad;
+};
+
+constexpr int kPersistWG = 512;
+constexpr int kPersistLdsFloats = 34816;        // 139264 bytes
+constexpr int kClsPass = 4096;                  // + 2*inverse + first
+constexpr int kPassLogN = 16, kPassLogR = 8, kPassLogCB = 5;
+constexpr int kPassTiles = (1 << (kPassLogN - kPassLogR)) >> kPassLogCB;
+constexpr int kSpinLimit = 1 << 22;
+// control words: [0] queue head, [1] error flags, [2] workgroups finished, [3..] deps
+constexpr int kCtlHeader = 3;
+
+__host__ __device__ constexpr int persist_logp(int logc, int var)
+{
+    return logc <= 4 ? logc : logc <= 13 ? (var ? 5 : 4) : (logc == 14 ? 5 : 6);
+}
+
+__host__ __device__ constexpr int lds_cls(int kind, int logc, int var) { return (kind * 16 + logc) * 2 + var; }
+
+template <int KIND, int LOGC, int VAR>
+__device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
+                                                  const float2* __restrict__ tw, float* lds)
+{
+    constexpr int LP = persist_logp(LOGC, VAR);
+    constexpr bool SPLIT = LOGC >= 15;
+    using SH = LdsShape<LOGC, LP, kPersistWG, KIND, SPLIT>;
+    static_assert(SH::LDS_WORDS <= kPersistLdsFloats, "class does not fit the persistent LDS buffer");
+    // An opaque copy of the thread index: without it the compiler shares the (identical)
+    // address arithmetic of the kernel's ~100 class bodies, hoists it above the dispatch
+    // and keeps it live through every body, which spills the large classes.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int slot = tid / SH::T;
+    const bool active = slot < w.count;
+    const BatchTransform d = trs[w.first + (active ? slot : 0)];
+    lds_transform<LOGC, LP, kPersistWG, KIND, SPLIT>(d.in, d.out, active, tw, lds, tid);
+}
+
+// Classes whose two variants coincide (n <= 16 or n >= 2^14) are instantiated once.
+template <int KIND, int LOGC>
+__device__ __forceinline__ int run_lds_var(int var, const BatchItem& w, const BatchTransform* __restrict__ trs,
+                                           const float2* __restrict__ tw, float* lds)
+{
+    if constexpr (persist_logp(LOGC, 0) == persist_logp(LOGC, 1)) {
+        run_lds_item<KIND, LOGC, 0>(w, trs, tw, lds);
+    } else {
+        if (var)
+            run_lds_item<KIND, LOGC, 1>(w, trs, tw, lds);
+        else
+            run_lds_item<KIND, LOGC, 0>(w, trs, tw, lds);
+    }
+    return 0;
+}
+
+template <int KIND, int LO, int... L>
+__device__ __forceinline__ void dispatch_lds(int logc, int var, const BatchItem& w,
+                                             const BatchTransform* __restrict__ trs,
+                                             const float2* __restrict__ tw, float* lds,
+                                             std::integer_sequence<int, L...>)
+{
+    (void)((logc == LO + L ? run_lds_var<KIND, LO + L>(var, w, trs, tw, lds) : 0), ...);
+}
+
+template <bool INV, bool FIRST>
+__device__ __forceinline__ void run_pass_item(const BatchItem& w, const BatchTransform* __restrict__ trs, int* ctl,
+                                              const DeviceTables& tb, float* lds)
+{
+    const BatchTransform d = trs[w.first];
+    int* dep = ctl + kCtlHeader + d.dep;
+    if constexpr (!FIRST) {
+        if (threadIdx.x == 0) {
+            int spins = 0;
+            while (__hip_atomic_load(dep, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < kPassTiles) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kSpinLimit) {          // never expected: flag and move on
+                    atomicOr(ctl + 1, 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    pass_tile<kPassLogR, kPassLogCB, INV, FIRST>(FIRST ? d.in : d.scratch, FIRST ? d.scratch : d.out, w.tile,
+                                                  kPassLogN, FIRST ? 0 : kPassLogR, tb.pyramid, tb.fine, lds,
+                                                  threadIdx.x);
+    if constexpr (FIRST) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(dep, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchItem* __restrict__ items, int n_items,
+                                                                     const BatchTransform* __restrict__ trs,
+                                                                     int* __restrict__ ctl, int n_deps,
+                                                                     DeviceTables tb)
+{
+    __shared__ float lds[kPersistLdsFloats];
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(ctl, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const BatchItem w = items[it];
+        if (w.cls >= kClsPass) {
+            switch (w.cls - kClsPass) {
+            case 0: run_pass_item<false, false>(w, trs, ctl, tb, lds); break;
+            case 1: run_pass_item<false, true>(w, trs, ctl, tb, lds); break;
+            case 2: run_pass_item<true, false>(w, trs, ctl, tb, lds); break;
+            default: run_pass_item<true, true>(w, trs, ctl, tb, lds); break;
+            }
+        } else {
+            const int var = w.cls & 1;
+            const int kind = (w.cls >> 1) >> 4;
+            const int logc = (w.cls >> 1) & 15;
+            switch (kind) {
+            case kComplexForward:
+                dispatch_lds<kComplexForward, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+                break;
+            case kComplexInverse:
+                dispatch_lds<kComplexInverse, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+                break;
+            case kRealForward:
+                dispatch_lds<kRealForward, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+                break;
+            default:
+                dispatch_lds<kRealInverse, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+                break;
+            }
+        }
+        __syncthreads();   // the LDS buffer is reused by the next item
+    }
+    // the last workgroup out resets the control words for the next execution
+    if (tid == 0) {
+        const int prev = atomicAdd(ctl + 2, 1);
+        if (prev == (int)gridDim.x - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            for (int i = 0; i < n_deps; ++i) ctl[kCtlHeader + i] = 0;
+            ctl[0] = 0;
+            ctl[2] = 0;
+        }
+    }
+}
+
+}  // namespace ckfft_gpu
+
+// ---- host side ----------------------------------------------------------------------
+
+using namespace ckfft_gpu;
+
+namespace {
+
+thread_local char g_batch_error[256] = "no error";
+
+bool berr(hipError_t e, const char* where)
+{
+    if (e == hipSuccess) return true;
+    std::snprintf(g_batch_error, sizeof(g_batch_error), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+    return false;
+}
+
+bool pow2_positive(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+bool is_device_pointer(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// bytes one transform reads and writes
+long long transform_bytes(int kind, int log2n)
+{
+    const long long n = 1ll << log2n;
+    return (kind == kRealForward || kind == kRealInverse) ? 4 * n + 8 * (n / 2 + 1) : 16 * n;
+}
+
+}  // namespace
+
+struct _CkFftBatchPlan {
+    CkFftContext* ctx;
+    int n_items = 0;
+    int n_trs = 0;
+    int n_deps = 0;
+    int grid = 0;
+    BatchItem* d_items = nullptr;
+    BatchTransform* d_trs = nullptr;
+    int* d_ctl = nullptr;
+    float2* d_scratch = nullptr;
+    struct Extra {
+        Kind kind;
+        int log2n;
+        const void* in;
+        void* out;
+    };
+    std::vector<Extra> extras;
+    bool use_graph = false;
+    bool graph_ready = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    CkFftBatchInfo info{};
+};
+
+namespace {
+
+// Build a plan; var_override >= 0 forces one variant for every class (the autotuner).
+CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int flags, int var_override)
+{
+    if (!ctx || ctx->magic != kContextMagic || count < 0 || (count > 0 && !descs)) return nullptr;
+    auto* plan = new (std::nothrow) CkFftBatchPlan();
+    if (!plan) return nullptr;
+    plan->ctx = ctx;
+    plan->use_graph = (flags & kCkFftBatch_Graph) != 0;
+
+    struct Entry {
+        int kind, log2n, logc, idx;
+    };
+    std::vector<Entry> pers;
+    for (int i = 0; i < count; ++i) {
+        const CkFftBatchDesc& d = descs[i];
+        if (d.kind < 0 || d.kind > 3 || !pow2_positive(d.n) || d.n > ctx->max_count) goto bad;
+        {
+            const int need_dir = (d.kind == kComplexForward || d.kind == kRealForward) ? kCkFftDirection_Forward
+                                                                                       : kCkFftDirection_Inverse;
+            if (!(ctx->direction & need_dir)) goto bad;
+        }
+        if (!d.input || !d.output || d.input == d.output) goto bad;
+        if (!is_device_pointer(d.input) || !is_device_pointer(d.output)) goto bad;
+        {
+            const int log2n = ilog2(d.n);
+            if (log2n > kLogMaxSupported || log2n > ctx->log_max) goto bad;
+            const bool real = d.kind >= kRealForward;
+            const int logc = real ? log2n - 1 : log2n;
+            const bool ok = real ? (log2n >= 3 && log2n <= kLogMaxLds + 1) : (log2n >= 1 && log2n <= kPassLogN);
+            if (ok)
+                pers.push_back({d.kind, log2n, logc, i});
+            else
+                plan->extras.push_back({(Kind)d.kind, log2n, d.input, d.output});
+        }
+    }
+    {
+        // group by class, keep descriptor order within a class
+        std::stable_sort(pers.begin(), pers.end(), [](const Entry& a, const Entry& b) {
+            return a.kind != b.kind ? a.kind < b.kind : a.logc < b.logc;
+        });
+        std::vector<BatchTransform> trs(pers.size());
+        std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item)
+        std::vector<BatchItem> first_pass;
+        int n_deps = 0;
+        for (size_t i = 0; i < pers.size(); ++i) {
+            const CkFftBatchDesc& d = descs[pers[i].idx];
+            trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr, -1, 0};
+            if (pers[i].logc == kPassLogN) trs[i].dep = n_deps++;
+        }
+        const size_t scratch_pts = (size_t)n_deps << kPassLogN;
+        if (n_deps && !berr(hipMalloc(&plan->d_scratch, sizeof(float2) * scratch_pts), "hipMalloc(batch scratch)"))
+            goto bad;
+        for (size_t i = 0; i < pers.size();) {
+            const Entry& e = pers[i];
+            if (e.logc == kPassLogN) {
+                trs[i].scratch = plan->d_scratch + ((size_t)trs[i].dep << kPassLogN);
+                const int inv = e.kind == kComplexInverse ? 1 : 0;
+                for (int q = 0; q < kPassTiles; ++q) {
+                    first_pass.push_back(BatchItem{kClsPass + 2 * inv + 1, (int)i, 1, q});
+                    work.push_back({2ll * (8 << kPassLogR) << kPassLogCB, BatchItem{kClsPass + 2 * inv, (int)i, 1, q}});
+                }
+                ++i;
+                continue;
+            }
+            const int var = var_override >= 0 ? var_override : ctx->tune[e.kind][e.logc];
+            const int T = 1 << (e.logc - persist_logp(e.logc, var));
+            const int tpw = kPersistWG / T;
+            size_t j = i;
+            while (j < pers.size() && pers[j].kind == e.kind && pers[j].logc == e.logc && (int)(j - i) < tpw) ++j;
+            const int cnt = (int)(j - i);
+            work.push_back({cnt * transform_bytes(e.kind, e.log2n),
+                            BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
+            i = j;
+        }
+        std::stable_sort(work.begin(), work.end(),
+                         [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
+                             return a.first > b.first;
+                         });
+        std::vector<BatchItem> items = first_pass;
+        for (auto& w : work) items.push_back(w.second);
+        plan->n_items = (int)items.size();
+        plan->n_trs = (int)trs.size();
+        plan->n_deps = n_deps;
+        if (plan->n_items) {
+            if (!berr(hipMalloc(&plan->d_items, sizeof(BatchItem) * items.size()), "hipMalloc(items)") ||
+                !berr(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * trs.size()), "hipMalloc(transforms)") ||
+                !berr(hipMalloc(&plan->d_ctl, sizeof(int) * (kCtlHeader + n_deps)), "hipMalloc(control)") ||
+                !berr(hipMemcpy(plan->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice),
+                      "upload items") ||
+                !berr(hipMemcpy(plan->d_trs, trs.data(), sizeof(BatchTransform) * trs.size(), hipMemcpyHostToDevice),
+                      "upload transforms") ||
+                !berr(hipMemset(plan->d_ctl, 0, sizeof(int) * (kCtlHeader + n_deps)), "clear control"))
+                goto bad;
+            int per_cu = 0, cus = 0;
+            if (!berr(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_persistent_kernel, kPersistWG, 0),
+                      "occupancy") ||
+                !berr(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count"))
+                goto bad;
+            plan->grid = std::max(1, std::min(per_cu * cus, plan->n_items));
+        }
+        plan->info.transforms = count;
+        plan->info.persistent_transforms = (int)pers.size();
+        plan->info.work_items = plan->n_items;
+        plan->info.workgroups = plan->grid;
+        plan->info.extra_launches = (int)plan->extras.size();
+        long long bytes = 0;
+        for (int i = 0; i < count; ++i) bytes += transform_bytes(descs[i].kind, ilog2(descs[i].n));
+        plan->info.algorithmic_bytes = bytes;
+    }
+    return plan;
+bad:
+    CkFftBatchPlanDestroy(plan);
+    return nullptr;
+}
+
+hipError_t launch_persistent(CkFftBatchPlan* p, hipStream_t s)
+{
+    const CkFftContext* c = p->ctx;
+    DeviceTables tb{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+    hipLaunchKernelGGL(fft_persistent_kernel, dim3(p->grid), dim3(kPersistWG), 0, s, p->d_items, p->n_items, p->d_trs,
+                       p->d_ctl, p->n_deps, tb);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+CkFftBatchPlan* CkFftBatchPlanCreate(CkFftContext* context, const CkFftBatchDesc* descs, int count, int flags)
+{
+    return plan_create(context, descs, count, flags, -1);
+}
+
+int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream)
+{
+    if (!plan) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->n_items) {
+        if (plan->use_graph) {
+            if (!plan->graph_ready) {
+                hipStream_t cap = nullptr;
+                if (!berr(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream")) return 0;
+                bool ok = berr(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture") &&
+                          berr(launch_persistent(plan, cap), "capture launch") &&
+                          berr(hipStreamEndCapture(cap, &plan->graph), "end capture") &&
+                          berr(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0), "instantiate");
+                (void)hipStreamDestroy(cap);
+                if (!ok) return 0;
+                plan->graph_ready = true;
+            }
+            if (!berr(hipGraphLaunch(plan->gexec, s), "graph launch")) return 0;
+        } else if (!berr(launch_persistent(plan, s), "persistent launch")) {
+            return 0;
+        }
+    }
+    const CkFftContext* c = plan->ctx;
+    DeviceTables tb{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+    for (const auto& e : plan->extras)
+        if (!berr(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, s), "extra launch")) return 0;
+    return 1;
+}
+
+int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info)
+{
+    if (!plan || !info) return 0;
+    *info = plan->info;
+    return 1;
+}
+
+int CkFftBatchPlanCheck(CkFftBatchPlan* plan)
+{
+    if (!plan) return 0;
+    if (!plan->d_ctl) return 1;
+    int flags = 0;
+    if (!berr(hipMemcpy(&flags, plan->d_ctl + 1, sizeof(int), hipMemcpyDeviceToHost), "read flags")) return 0;
+    if (flags) {
+        const int zero = 0;
+        (void)hipMemcpy(plan->d_ctl + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
+        std::snprintf(g_batch_error, sizeof(g_batch_error), "persistent kernel flagged a dependency timeout");
+        return 0;
+    }
+    return 1;
+}
+
+void CkFftBatchPlanDestroy(CkFftBatchPlan* plan)
+{
+    if (!plan) return;
+    if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->d_items) (void)hipFree(plan->d_items);
+    if (plan->d_trs) (void)hipFree(plan->d_trs);
+    if (plan->d_ctl) (void)hipFree(plan->d_ctl);
+    if (plan->d_scratch) (void)hipFree(plan->d_scratch);
+    delete plan;
+}
+
+const char* CkFftBatchLastError(void) { return g_batch_error; }
+
+// ---- autotuner ------------------------------------------------------------------------
+
+static const char* kKindTag[4] = {"cfwd", "cinv", "rfwd", "rinv"};
+
+int CkFftAutotune(CkFftContext* context, const char* cachePath)
+{
+    if (!context || context->magic != kContextMagic) return 0;
+    const size_t kBytesPerClass = 32u << 20;
+    void* in = nullptr;
+    void* out = nullptr;
+    if (!berr(hipMalloc(&in, kBytesPerClass), "hipMalloc(tune in)") ||
+        !berr(hipMalloc(&out, kBytesPerClass + 4096 * 8), "hipMalloc(tune out)") ||
+        !berr(hipMemset(in, 0, kBytesPerClass), "clear tune input")) {
+        if (in) (void)hipFree(in);
+        return 0;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    std::string json = "{\n \"arch\": \"gfx950\",\n \"choice\": {";
+    std::string times = "},\n \"us_per_call\": {";
+    bool first_entry = true;
+    int ok = 1;
+    for (int kind = 0; kind < 4 && ok; ++kind) {
+        const int need_dir = (kind == kComplexForward || kind == kRealForward) ? kCkFftDirection_Forward
+                                                                               : kCkFftDirection_Inverse;
+        if (!(context->direction & need_dir)) continue;
+        for (int logc = 5; logc <= 13; ++logc) {
+            const int log2n = kind >= kRealForward ? logc + 1 : logc;
+            if (log2n > context->log_max) continue;
+            const long long tb = transform_bytes(kind, log2n);
+            const int count = (int)std::max(1ll, (long long)(kBytesPerClass / 2) / tb);
+            const long long n = 1ll << log2n;
+            const long long in_stride = kind == kRealForward ? 4 * n : (kind == kRealInverse ? 8 * (n / 2 + 1) : 8 * n);
+            const long long out_stride = kind == kRealForward ? 8 * (n / 2 + 1) : (kind == kRealInverse ? 4 * n : 8 * n);
+            std::vector<CkFftBatchDesc> d(count);
+            for (int i = 0; i < count; ++i)
+                d[i] = CkFftBatchDesc{kind, (int)n, (const char*)in + i * in_stride, (char*)out + i * out_stride};
+            float best = 1e30f;
+            int best_var = 0;
+            float t_var[2] = {0, 0};
+            for (int var = 0; var < 2; ++var) {
+                CkFftBatchPlan* p = plan_create(context, d.data(), count, 0, var);
+                if (!p) {
+                    ok = 0;
+                    break;
+                }
+                for (int w = 0; w < 3; ++w) CkFftBatchPlanExecute(p, nullptr);
+                (void)hipEventRecord(e0, nullptr);
+                const int reps = 10;
+                for (int r = 0; r < reps; ++r) CkFftBatchPlanExecute(p, nullptr);
+                (void)hipEventRecord(e1, nullptr);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                CkFftBatchPlanDestroy(p);
+                t_var[var] = ms * 1000.f / reps;
+                if (t_var[var] < best) {
+                    best = t_var[var];
+                    best_var = var;
+                }
+            }
+            if (!ok) break;
+            context->tune[kind][logc] = (unsigned char)best_var;
+            char buf[160];
+            std::snprintf(buf, sizeof(buf), "%s\n  \"%s_%d\": %d", first_entry ? "" : ",", kKindTag[kind], log2n,
+                          best_var);
+            json += buf;
+            std::snprintf(buf, sizeof(buf), "%s\n  \"%s_%d\": [%.2f, %.2f]", first_entry ? "" : ",", kKindTag[kind],
+                          log2n, t_var[0], t_var[1]);
+            times += buf;
+            first_entry = false;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    if (!ok) return 0;
+    json += times + "}\n}\n";
+    if (cachePath) {
+        FILE* f = std::fopen(cachePath, "w");
+        if (!f) return 0;
+        std::fputs(json.c_str(), f);
+        std::fclose(f);
+    }
+    return 1;
+}
+
+int CkFftLoadTuning(CkFftContext* context, const char* cachePath)
+{
+    if (!context || context->magic != kContextMagic || !cachePath) return 0;
+    FILE* f = std::fopen(cachePath, "r");
+    if (!f) return 0;
+    std::string text;
+    char buf[4096];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    const size_t c0 = text.find("\"choice\"");
+    if (c0 == std::string::npos) return 0;
+    const size_t c1 = text.find('}', c0);
+    const std::string sec = text.substr(c0, c1 - c0);
+    int loaded = 0;
+    for (int kind = 0; kind < 4; ++kind)
+        for (int logc = 5; logc <= 13; ++logc) {
+            const int log2n = kind >= kRealForward ? logc + 1 : logc;
+            char key[32];
+            std::snprintf(key, sizeof(key), "\"%s_%d\":", kKindTag[kind], log2n);
+            const size_t k = sec.find(key);
+            if (k == std::string::npos) continue;
+            const int v = std::atoi(sec.c_str() + k + std::strlen(key));
+            if (v == 0 || v == 1) {
+                context->tune[kind][logc] = (unsigned char)v;
+                ++loaded;
+            }
+        }
+    return loaded > 0 ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,360 +1,28 @@
-// fft_lds.hip -- one-workgroup Stockham transforms for complex sizes up to 2^15.
-//
-// Replaces the reference's recursive radix-4 decimation-in-time transform
-// (src/ckfft/fft_default.cpp:12-266) and its real pack/unpack passes
-// (src/ckfft/fft_real_default.cpp:121-222) for every size that fits one workgroup.
-//
-// Design (gfx950, wave64):
-//  * Each transform of N complex points is owned by T = N/P threads; every thread
-//    keeps P points in registers for the whole transform.  Stages are radix 16
-//    (the first stage takes the remaining radix 2/4/8), so a 4096-point transform is
-//    three register-resident radix-16 stages with two LDS exchanges between them.
-//  * Stockham autosort indexing: stage s with radix R and span Ns reads element
-//    j + r*N/R and writes (j/Ns)*Ns*R + j%Ns + r*Ns, so the first stage reads global
-//    memory and the last stage writes it in natural order, both fully coalesced
-//    (consecutive lanes touch consecutive 8-byte elements).
-//  * LDS holds real and imaginary parts in separate padded planes (one pad word per
-//    16) so the strided first-stage writes do not collide on LDS banks.  When both
-//    planes do not fit (N = 2^15: 2 x 136 KiB), the exchange runs plane by plane
-//    through one 136 KiB buffer.
-//  * Twiddles come from the FP64-generated pyramid table (level L holds
-//    exp(-2 pi i k/L)); stage s uses level Ns*R, read through the caches.
-//  * Real forward (2N reals): an N-point complex transform of the samples viewed as
-//    complex, then the pack pass out[k] = A - i W^k D, out[N-k] = conj(A + i W^k D)
-//    with A = Z[k] + conj Z[N-k], D = Z[k] - conj Z[N-k], W = exp(-2 pi i/2N), fused
-//    after the last stage.  This is the reference's post-pass
-//    (src/ckfft/fft_real_default.cpp:129-170) written for one (k, N-k) pair per step.
-//  * Real inverse: the unpack pre-pass t[k] = (Y[k] + conj Y[N-k]) +
-//    i conj(W^k) (Y[k] - conj Y[N-k]) (src/ckfft/fft_real_default.cpp:180-219) is
-//    fused into the first stage's global loads, then the N-point inverse transform
-//    writes the 2N reals as N complex values.
-#include "fft_stockham.h"
+// fft_lds.hip -- batch kernels for complex sizes up to 2^15 (real up to 2^16): one
+// workgroup holds one or more whole transforms.  The transform body is in fft_lds.h.
+#include <utility>
+
+#include "fft_lds.h"
 
 namespace ckfft_gpu {
 
-// Which stage (if any) uses the mirror-closed assignment: the first stage of a real
-// inverse (unpack fused into its loads) and the last stage of a real forward (pack fused
-// into its stores), when that stage has an even number >= 2 of butterflies per thread.
-template <class PL, int KIND>
-constexpr int mirror_stage()
-{
-    if constexpr (KIND == kRealInverse) {
-        constexpr int B = bflies<PL, 0>();
-        return (B >= 2 && B % 2 == 0) ? 0 : -1;
-    } else if constexpr (KIND == kRealForward) {
-        constexpr int B = bflies<PL, PL::NSTAGES - 1>();
-        return (B >= 2 && B % 2 == 0) ? PL::NSTAGES - 1 : -1;
-    } else {
-        return -1;
-    }
-}
-
-template <class PL, int S, bool INV, bool SPLIT, int MS>
-__device__ __forceinline__ void run_stages(float2* v, int t, float* re, float* im,
-                                           const float2* __restrict__ tw)
-{
-    if constexpr (S < PL::NSTAGES) {
-        exchange<PL, S - 1, S, SPLIT, MS == S - 1, MS == S>(v, t, re, im);
-        stage_compute<PL, S, INV, MS == S>(v, t, tw);
-        run_stages<PL, S + 1, INV, SPLIT, MS>(v, t, re, im, tw);
-    }
-}
-
-// ---- first-stage loads ---------------------------------------------------------
-
-// First-stage loads.  They carry no per-element predicate: a workgroup slot past the
-// last signal reads a valid signal instead (see the kernel) and never stores, so every
-// load issues back to back.
-template <class PL>
-__device__ __forceinline__ void load_first_complex(float2* v, int t, const float2* __restrict__ src)
-{
-    constexpr int R = 1 << PL::rlog(0);
-    constexpr int B = PL::P / R;
-#pragma unroll
-    for (int b = 0; b < B; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = src[in_index<PL, 0>(t + b * PL::T, r)];
-}
-
-// Real-inverse unpack of one point: t[k] = (Y[k] + conj Y[N-k]) + i e^{+2 pi i k/2N} (Y[k] - conj Y[N-k]),
-// with w = e^{-2 pi i k/2N} from the table.
-__device__ __forceinline__ float2 unpack_one(float2 yk, float2 ym, float2 w)
-{
-    const float2 a = make_float2(yk.x + ym.x, yk.y - ym.y);     // Y[k] + conj Y[N-k]
-    const float2 d = make_float2(yk.x - ym.x, yk.y + ym.y);     // Y[k] - conj Y[N-k]
-    const float2 c = cmul(cconj(w), d);
-    return make_float2(a.x - c.y, a.y + c.x);                   // a + i c
-}
-
-// Real inverse, general assignment: every point reads Y[k] and Y[N-k] itself (the second
-// read hits the caches; used when the first stage has one butterfly per thread).
-template <class PL>
-__device__ __forceinline__ void load_first_unpack(float2* v, int t, const float2* __restrict__ y,
-                                                  const float2* __restrict__ tw)
-{
-    constexpr int N = PL::N;
-    constexpr int R = 1 << PL::rlog(0);
-    constexpr int B = PL::P / R;
-    const float2* lvl = tw + 2 * N;             // level 2N: exp(-2 pi i k / 2N)
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        // point j + r*N/R: W^(j + r N/R) = W^j * exp(-2 pi i r / 2R), the second a constant
-        const int j = t + b * PL::T;
-        const float2 wj = lvl[j];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int k = in_index<PL, 0>(j, r);
-            v[b * R + r] = unpack_one(y[k], y[N - k], cmul(wj, w64(r * (32 / R))));
-        }
-    }
-}
-
-// Real inverse with a mirror-closed first stage: each Y value is read once, by the
-// thread that also holds its mirror, and the pairs are unpacked in registers.  Element
-// (b, r) of a lower slot b < H is point k = jl + r*M and pairs with element (b+H, R-1-r),
-// point N - k.  Thread 0's slot pair holds the self-mirrored butterflies 0 and M/2
-// instead; it fetches its partners from memory (predicated loads in one lane).
-template <class PL>
-__device__ __forceinline__ void load_first_unpack_mirror(float2* v, int t, const float2* __restrict__ y,
-                                                         const float2* __restrict__ tw)
-{
-    constexpr int N = PL::N;
-    constexpr int R = 1 << PL::rlog(0);
-    constexpr int B = PL::P / R;
-    constexpr int H = B / 2;
-    constexpr int M = N / R;
-    const float2* lvl = tw + 2 * N;
-#pragma unroll
-    for (int b = 0; b < B; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b), r)];
-    const int jh0 = bfly<PL, 0, true>(t, H);            // M - t, or M/2 for t == 0
-#pragma unroll
-    for (int b = 0; b < H; ++b) {
-        const int jl = t + b * PL::T;
-        const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wj = lvl[jl];                       // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float2& lo = v[b * R + r];
-            float2& hi = v[(b + H) * R + R - 1 - r];
-            const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
-            const float2 w0 = cmul(wj, w64(r * (32 / R)));
-            float2 p0 = hi, p1 = lo;                     // Y[N - k0], Y[N - k1] when paired
-            float2 w1 = make_float2(-w0.x, w0.y);        // W^(N-k0) = -conj(W^k0)
-            if (b == 0 && t == 0) {
-                // butterfly M/2: W^(M/2 + (R-1-r)M) = exp(-2 pi i (2(R-1-r)+1) / 4R)
-                p0 = y[N - k0];
-                p1 = y[N - k1];
-                w1 = w64((2 * (R - 1 - r) + 1) * (16 / R));
-            }
-            const float2 y0 = lo, y1 = hi;
-            lo = unpack_one(y0, p0, w0);
-            hi = unpack_one(y1, p1, w1);
-        }
-    }
-}
-
-// ---- last-stage stores -----------------------------------------------------------
-
-template <class PL>
-__device__ __forceinline__ void store_last(const float2* v, int t, float2* __restrict__ dst, bool active)
-{
-    constexpr int S = PL::NSTAGES - 1;
-    constexpr int R = 1 << PL::rlog(S);
-    constexpr int B = PL::P / R;
-    if (!active) return;
-#pragma unroll
-    for (int b = 0; b < B; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
-}
-
-// Pack one (k, N-k) pair of the real forward transform; zk = Z[k], zm = Z[N-k mod N].
-template <int N>
-__device__ __forceinline__ void pack_pair(int k, float2 zk, float2 zm, const float2* __restrict__ lvl,
-                                          float2* __restrict__ dst)
-{
-    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);
-    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
-    const float2 c = cmul(lvl[k], d);                       // W^k D
-    dst[k] = make_float2(a.x + c.y, a.y - c.x);             // a - i c
-    if (k != N - k) dst[N - k] = make_float2(a.x - c.y, -(a.y + c.x));   // conj(a + i c)
-}
-
-// One real-forward output: out[k] = A - i W^k D, A = Z[k] + conj Z[N-k], D = Z[k] - conj Z[N-k].
-__device__ __forceinline__ float2 pack_one(float2 zk, float2 zm, float2 w)
-{
-    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);
-    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
-    const float2 c = cmul(w, d);
-    return make_float2(a.x + c.y, a.y - c.x);
-}
-
-// Real forward epilogue with a mirror-closed last stage: every output point k is written
-// once, by the thread that holds Z[k], from Z[k] and its mirror Z[N-k] in the same
-// thread's registers.  Thread 0's first slot pair holds the self-mirrored butterflies 0
-// and M/2, whose partners sit at other register positions; it also writes out[N].
-template <class PL>
-__device__ __forceinline__ void pack_store_mirror(const float2* v, int t, const float2* __restrict__ tw,
-                                                  float2* __restrict__ dst, bool active)
-{
-    constexpr int S = PL::NSTAGES - 1;
-    constexpr int N = PL::N;
-    constexpr int R = 1 << PL::rlog(S);
-    constexpr int B = PL::P / R;
-    constexpr int H = B / 2;
-    constexpr int M = N / R;                    // the last stage writes point j + r*M
-    const float2* lvl = tw + 2 * N;
-    if (!active) return;
-    const int jh0 = bfly<PL, S, true>(t, H);    // M - t, or M/2 for t == 0
-#pragma unroll
-    for (int b = 0; b < H; ++b) {
-        const int jl = t + b * PL::T;
-        const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wj = lvl[jl];               // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float2 lo = v[b * R + r];
-            const float2 hi = v[(b + H) * R + R - 1 - r];
-            const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
-            const float2 w0 = cmul(wj, w64(r * (32 / R)));
-            float2 p0 = hi, p1 = lo;
-            float2 w1 = make_float2(-w0.x, w0.y);
-            if (b == 0) {
-                // thread 0: point r*M pairs with (R-r)*M, point M/2 + (R-1-r)*M with M/2 + r*M,
-                // whose twiddle is exp(-2 pi i (2(R-1-r)+1) / 4R)
-                const bool z = (t == 0);
-                p0 = z ? v[(R - r) & (R - 1)] : p0;
-                p1 = z ? v[H * R + r] : p1;
-                w1 = z ? w64((2 * (R - 1 - r) + 1) * (16 / R)) : w1;
-            }
-            dst[k0] = pack_one(lo, p0, w0);
-            dst[k1] = pack_one(hi, p1, w1);
-        }
-    }
-    if (t == 0) dst[N] = pack_one(v[0], v[0], make_float2(-1.f, 0.f));   // the Nyquist output
-}
-
-// Real forward epilogue when the last stage cannot be mirror-closed: Z goes through
-// LDS, then thread t packs pairs k = t + q*T for k in [0, N/2].
-template <class PL, bool SPLIT>
-__device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float* im,
-                                              const float2* __restrict__ tw, float2* __restrict__ dst,
-                                              bool active)
-{
-    constexpr int N = PL::N;
-    constexpr int T = PL::T;
-    constexpr int S = PL::NSTAGES - 1;
-    constexpr int Q = (N / 2) / T + 1;                      // pair slots per thread
-    const float2* lvl = tw + 2 * N;
-    float2 zk[Q], zm[Q];
-    if constexpr (!SPLIT) {
-        lds_write<PL, S, 3>(v, t, re, im);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int k = t + q * T;
-            if (k <= N / 2) {
-                const int ik = lds_pad(k & (N - 1)), im_ = lds_pad((N - k) & (N - 1));
-                zk[q] = make_float2(re[ik], im[ik]);
-                zm[q] = make_float2(re[im_], im[im_]);
-            }
-        }
-    } else {
-        lds_write<PL, S, 1>(v, t, re, im);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int k = t + q * T;
-            if (k <= N / 2) {
-                zk[q].x = re[lds_pad(k & (N - 1))];
-                zm[q].x = re[lds_pad((N - k) & (N - 1))];
-            }
-        }
-        __syncthreads();
-        lds_write<PL, S, 2>(v, t, re, im);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int k = t + q * T;
-            if (k <= N / 2) {
-                zk[q].y = im[lds_pad(k & (N - 1))];
-                zm[q].y = im[lds_pad((N - k) & (N - 1))];
-            }
-        }
-    }
-    if (!active) return;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int k = t + q * T;
-        if (k <= N / 2) pack_pair<N>(k, zk[q], zm[q], lvl, dst);
-    }
-}
-
 // ---- the kernel ----------------------------------------------------------------------
 
-// KIND: kComplexForward / kComplexInverse / kRealForward / kRealInverse.
-// For the real kinds N = 2^LOGN is half the real length.  The real forward plan puts
-// its small radix last so that the last stage can be mirror-closed.
+// `count` contiguous signals; workgroup slot g handles signal blockIdx.x * TPW + slot.
 template <int LOGN, int LOGP, int WG, int KIND, bool SPLIT>
 __global__ __launch_bounds__(WG) void fft_lds_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                      const float2* __restrict__ tw, int count)
 {
-    using PL = Plan<LOGN, LOGP, KIND == kRealForward>;
-    constexpr bool INV = (KIND == kComplexInverse || KIND == kRealInverse);
-    constexpr int N = PL::N;
-    constexpr int T = PL::T;
-    constexpr int TPW = WG / T;                 // transforms per workgroup
-    static_assert(TPW >= 1 && TPW * T == WG, "workgroup must hold whole transforms");
-    static_assert(!SPLIT || TPW == 1, "plane-by-plane exchange is for one transform per workgroup");
-    constexpr bool NEED_LDS = (T > 1);
-    constexpr int SPAN = lds_span(N);
-    constexpr int LDS_WORDS = NEED_LDS ? (SPLIT ? SPAN : 2 * SPAN * TPW) : 1;
-    constexpr int MS = mirror_stage<PL, KIND>();
-    __shared__ float lds[LDS_WORDS];
-
-    const int tr = threadIdx.x / T;
-    const int t = threadIdx.x - tr * T;
-    const long long g = (long long)blockIdx.x * TPW + tr;
+    using SH = LdsShape<LOGN, LOGP, WG, KIND, SPLIT>;
+    __shared__ float lds[SH::LDS_WORDS];
+    const int tr = threadIdx.x / SH::T;
+    const long long g = (long long)blockIdx.x * SH::TPW + tr;
     const bool active = g < count;
-    float* re = lds + (SPLIT ? 0 : tr * 2 * SPAN);
-    float* im = SPLIT ? lds : re + SPAN;
-
     // slots past the last signal (only in the last workgroup) compute on the last
     // signal and store nothing
     const long long gl = active ? g : (long long)count - 1;
-    float2 v[PL::P];
-    if constexpr (KIND == kRealInverse) {
-        if constexpr (MS == 0)
-            load_first_unpack_mirror<PL>(v, t, in + gl * (N + 1), tw);
-        else
-            load_first_unpack<PL>(v, t, in + gl * (N + 1), tw);
-    } else {
-        load_first_complex<PL>(v, t, in + gl * N);
-    }
-
-    stage_compute<PL, 0, INV, MS == 0>(v, t, tw);
-    run_stages<PL, 1, INV, SPLIT, MS>(v, t, re, im, tw);
-
-    if constexpr (KIND == kRealForward) {
-        float2* dst = out + g * (N + 1);
-        if constexpr (T == 1) {
-            // whole transform in this thread's registers, in natural order
-            if (active) {
-                const float2* lvl = tw + 2 * N;
-#pragma unroll
-                for (int k = 0; k <= N / 2; ++k) pack_pair<N>(k, v[k & (N - 1)], v[(N - k) & (N - 1)], lvl, dst);
-            }
-        } else if constexpr (MS == PL::NSTAGES - 1) {
-            pack_store_mirror<PL>(v, t, tw, dst, active);
-        } else {
-            pack_epilogue<PL, SPLIT>(v, t, re, im, tw, dst, active);
-        }
-    } else {
-        store_last<PL>(v, t, out + g * N, active);
-    }
+    lds_transform<LOGN, LOGP, WG, KIND, SPLIT>(in + gl * SH::IN_STRIDE, out + gl * SH::OUT_STRIDE, active, tw, lds,
+                                               threadIdx.x);
 }
 
 // ---- host-side dispatch ------------------------------------------------------------

@@ -1,0 +1,134 @@
+// fft_pass.h -- one tile of a multi-pass (global-memory Stockham) stage, shared by the
+// pass kernels (fft_pass.hip) and the persistent kernel (fft_batch.hip).
+#pragma once
+#include "fft_stockham.h"
+
+namespace ckfft_gpu {
+
+// exp(-2 pi i m / 2^loglev) from the device tables: the pyramid up to 2^16, above that
+// a coarse pyramid entry times a fine-table entry (both FP64-rounded; one product).
+__device__ __forceinline__ float2 twiddle_at(const float2* __restrict__ pyr, const float2* __restrict__ fine,
+                                             int loglev, long long m)
+{
+    if (loglev <= kLogMaxPyramid) return pyr[(1ll << loglev) + m];
+    const float2 c = pyr[(1ll << (loglev - kLogFine)) + (m >> kLogFine)];
+    const float2 f = fine[((long long)(loglev - kLogMaxPyramid - 1) << kLogFine) + (m & ((1 << kLogFine) - 1))];
+    return cmul(c, f);
+}
+
+// Tile order: contiguous tile ranges per XCD (workgroup i runs on XCD i mod 8).
+__device__ __forceinline__ long long xcd_tile(long long bid, long long nblocks)
+{
+    const long long per = nblocks >> 3;
+    if (bid >= (per << 3)) return bid;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+// Tile width of the stand-alone pass kernels: 16 butterflies (128-byte runs) up to radix
+// 2^10, fewer above so that a tile holds <= 16384 points, more below so that a
+// workgroup has >= 256 threads.
+constexpr int default_logcb(int logr) { return logr >= 8 ? (logr <= 10 ? 4 : 14 - logr) : 12 - logr; }
+
+template <int LOGR, int LOGCB_ = default_logcb(LOGR)>
+struct PassShape {
+    static constexpr int LOGP = 4;                                        // 16 points per thread
+    static constexpr int R = 1 << LOGR;
+    static constexpr int T = R >> LOGP;                                   // threads per DFT
+    static constexpr int LOGCB = LOGCB_;
+    static constexpr int CB = 1 << LOGCB;
+    static constexpr int WG = CB * T;
+    static constexpr int SPAN = lds_span(R);
+    static constexpr int RSTRIDE = 2 * SPAN + 1;   // per-DFT LDS region; odd so regions start on different banks
+};
+
+template <class PL, int S, bool INV, bool FIRST, class SH>
+__device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float* imA, int tB, float* reB,
+                                            float* imB, const float2* __restrict__ pyr)
+{
+    if constexpr (S < PL::NSTAGES) {
+        // the first pass stores with "point-fastest" lanes (contiguous output block); so the
+        // last sub-stage switches mapping B; every other sub-stage uses "DFT-fastest" lanes (A)
+        constexpr bool B_LAST = FIRST && (S == PL::NSTAGES - 1);
+        lds_write<PL, S - 1, 3>(v, tA, reA, imA);
+        __syncthreads();
+        if constexpr (B_LAST) {
+            lds_read<PL, S, 3>(v, tB, reB, imB);
+        } else {
+            lds_read<PL, S, 3>(v, tA, reA, imA);
+        }
+        __syncthreads();
+        stage_compute<PL, S, INV>(v, B_LAST ? tB : tA, pyr);
+        pass_stages<PL, S + 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
+    }
+}
+
+// One tile of one pass: butterflies q*CB .. q*CB + CB-1 of the signal at `src` (2^logn
+// points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
+// `lds` holds CB * RSTRIDE floats.  Used by the pass kernel below and by the persistent
+// kernel.
+template <int LOGR, int LOGCB, bool INV, bool FIRST>
+__device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
+                                          int logn, int logns, const float2* __restrict__ pyr,
+                                          const float2* __restrict__ fine, float* lds, int tid)
+{
+    using SH = PassShape<LOGR, LOGCB>;
+    using PL = Plan<LOGR, SH::LOGP>;
+    static_assert(PL::NSTAGES >= 2, "a pass needs at least two sub-stages");
+    constexpr int R = SH::R, T = SH::T, CB = SH::CB;
+    const long long N = 1ll << logn;
+    const long long stride = N >> LOGR;                     // N/R
+    const long long j0 = q * CB;
+
+    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
+    const int cB = tid / T, tB = tid - cB * T;             // point-fastest lanes
+    float* reA = lds + cA * SH::RSTRIDE;
+    float* imA = reA + SH::SPAN;
+    float* reB = lds + cB * SH::RSTRIDE;
+    float* imB = reB + SH::SPAN;
+    constexpr int R0 = 1 << PL::rlog(0);
+    constexpr int B0 = PL::P / R0;
+    float2 v[PL::P];
+#pragma unroll
+    for (int b = 0; b < B0; ++b)
+#pragma unroll
+        for (int rr = 0; rr < R0; ++rr) {
+            const int r = in_index<PL, 0>(tA + b * T, rr);
+            v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
+        }
+    if constexpr (!FIRST) {
+        const long long jm = (j0 + cA) & ((1ll << logns) - 1);
+        const int loglev = logns + LOGR;
+#pragma unroll
+        for (int b = 0; b < B0; ++b)
+#pragma unroll
+            for (int rr = 0; rr < R0; ++rr) {
+                const int r = in_index<PL, 0>(tA + b * T, rr);
+                v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)r * jm)));
+            }
+    }
+    stage_compute<PL, 0, INV>(v, tA, pyr);
+    pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
+
+    constexpr int SL = PL::NSTAGES - 1;
+    constexpr int RL = 1 << PL::rlog(SL);
+    constexpr int BL = PL::P / RL;
+    if constexpr (FIRST) {
+        // span 1: butterfly j writes points j*R .. j*R + R-1, one contiguous block per tile
+        float2* blk = dst + (j0 + cB) * R;
+#pragma unroll
+        for (int b = 0; b < BL; ++b)
+#pragma unroll
+            for (int rr = 0; rr < RL; ++rr) blk[out_index<PL, SL>(tB + b * T, rr)] = v[b * RL + rr];
+    } else {
+        const long long ns = 1ll << logns;
+        const long long j = j0 + cA;
+        float2* base = dst + (j >> logns) * ns * R + (j & (ns - 1));
+#pragma unroll
+        for (int b = 0; b < BL; ++b)
+#pragma unroll
+            for (int rr = 0; rr < RL; ++rr)
+                base[(long long)out_index<PL, SL>(tA + b * T, rr) * ns] = v[b * RL + rr];
+    }
+}
+
+}  // namespace ckfft_gpu

@@ -17,62 +17,11 @@
 // Real transforms above 2^16 samples run the half-size complex transform here plus a
 // separate pack (forward) or unpack (inverse) pass; below that they stay fused in the
 // one-workgroup kernels.
-#include "fft_stockham.h"
+#include <utility>
+
+#include "fft_pass.h"
 
 namespace ckfft_gpu {
-
-// exp(-2 pi i m / 2^loglev) from the device tables: the pyramid up to 2^16, above that
-// a coarse pyramid entry times a fine-table entry (both FP64-rounded; one product).
-__device__ __forceinline__ float2 twiddle_at(const float2* __restrict__ pyr, const float2* __restrict__ fine,
-                                             int loglev, long long m)
-{
-    if (loglev <= kLogMaxPyramid) return pyr[(1ll << loglev) + m];
-    const float2 c = pyr[(1ll << (loglev - kLogFine)) + (m >> kLogFine)];
-    const float2 f = fine[((long long)(loglev - kLogMaxPyramid - 1) << kLogFine) + (m & ((1 << kLogFine) - 1))];
-    return cmul(c, f);
-}
-
-// Tile order: contiguous tile ranges per XCD (workgroup i runs on XCD i mod 8).
-__device__ __forceinline__ long long xcd_tile(long long bid, long long nblocks)
-{
-    const long long per = nblocks >> 3;
-    if (bid >= (per << 3)) return bid;
-    return (bid & 7) * per + (bid >> 3);
-}
-
-template <int LOGR>
-struct PassShape {
-    static constexpr int LOGP = 4;                                        // 16 points per thread
-    static constexpr int R = 1 << LOGR;
-    static constexpr int T = R >> LOGP;                                   // threads per DFT
-    static constexpr int LOGCB = LOGR >= 8 ? (LOGR <= 10 ? 4 : 14 - LOGR) // <= 16384 points per workgroup
-                                           : 12 - LOGR;                   // >= 256 threads
-    static constexpr int CB = 1 << LOGCB;
-    static constexpr int WG = CB * T;
-    static constexpr int SPAN = lds_span(R);
-    static constexpr int RSTRIDE = 2 * SPAN + 1;   // per-DFT LDS region; odd so regions start on different banks
-};
-
-template <class PL, int S, bool INV, bool FIRST, class SH>
-__device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float* imA, int tB, float* reB,
-                                            float* imB, const float2* __restrict__ pyr)
-{
-    if constexpr (S < PL::NSTAGES) {
-        // the first pass stores with "point-fastest" lanes (contiguous output block); so the
-        // last sub-stage switches mapping B; every other sub-stage uses "DFT-fastest" lanes (A)
-        constexpr bool B_LAST = FIRST && (S == PL::NSTAGES - 1);
-        lds_write<PL, S - 1, 3>(v, tA, reA, imA);
-        __syncthreads();
-        if constexpr (B_LAST) {
-            lds_read<PL, S, 3>(v, tB, reB, imB);
-        } else {
-            lds_read<PL, S, 3>(v, tA, reA, imA);
-        }
-        __syncthreads();
-        stage_compute<PL, S, INV>(v, B_LAST ? tB : tA, pyr);
-        pass_stages<PL, S + 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
-    }
-}
 
 template <int LOGR, bool INV, bool FIRST>
 __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
@@ -80,72 +29,12 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const float2* __restrict__ fine, int logn, int logns, long long tiles_per_signal, long long total_tiles)
 {
     using SH = PassShape<LOGR>;
-    using PL = Plan<LOGR, SH::LOGP>;
-    static_assert(PL::NSTAGES >= 2, "a pass needs at least two sub-stages");
-    constexpr int R = SH::R, T = SH::T, CB = SH::CB;
-    __shared__ float lds[CB * SH::RSTRIDE];
-
+    __shared__ float lds[SH::CB * SH::RSTRIDE];
     const long long tile = xcd_tile(blockIdx.x, total_tiles);
     const long long g = tile / tiles_per_signal;
     const long long q = tile - g * tiles_per_signal;
     const long long N = 1ll << logn;
-    const long long stride = N >> LOGR;                     // N/R
-    const long long j0 = q * CB;
-    const float2* src = in + g * N;
-    float2* dst = out + g * N;
-
-    const int tid = threadIdx.x;
-    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
-    const int cB = tid / T, tB = tid - cB * T;             // point-fastest lanes
-    float* reA = lds + cA * SH::RSTRIDE;
-    float* imA = reA + SH::SPAN;
-    float* reB = lds + cB * SH::RSTRIDE;
-    float* imB = reB + SH::SPAN;
-
-    constexpr int R0 = 1 << PL::rlog(0);
-    constexpr int B0 = PL::P / R0;
-    float2 v[PL::P];
-#pragma unroll
-    for (int b = 0; b < B0; ++b)
-#pragma unroll
-        for (int rr = 0; rr < R0; ++rr) {
-            const int r = in_index<PL, 0>(tA + b * T, rr);
-            v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
-        }
-    if constexpr (!FIRST) {
-        const long long jm = (j0 + cA) & ((1ll << logns) - 1);
-        const int loglev = logns + LOGR;
-#pragma unroll
-        for (int b = 0; b < B0; ++b)
-#pragma unroll
-            for (int rr = 0; rr < R0; ++rr) {
-                const int r = in_index<PL, 0>(tA + b * T, rr);
-                v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)r * jm)));
-            }
-    }
-    stage_compute<PL, 0, INV>(v, tA, pyr);
-    pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
-
-    constexpr int SL = PL::NSTAGES - 1;
-    constexpr int RL = 1 << PL::rlog(SL);
-    constexpr int BL = PL::P / RL;
-    if constexpr (FIRST) {
-        // span 1: butterfly j writes points j*R .. j*R + R-1, one contiguous block per tile
-        float2* blk = dst + (j0 + cB) * R;
-#pragma unroll
-        for (int b = 0; b < BL; ++b)
-#pragma unroll
-            for (int rr = 0; rr < RL; ++rr) blk[out_index<PL, SL>(tB + b * T, rr)] = v[b * RL + rr];
-    } else {
-        const long long ns = 1ll << logns;
-        const long long j = j0 + cA;
-        float2* base = dst + (j >> logns) * ns * R + (j & (ns - 1));
-#pragma unroll
-        for (int b = 0; b < BL; ++b)
-#pragma unroll
-            for (int rr = 0; rr < RL; ++rr)
-                base[(long long)out_index<PL, SL>(tA + b * T, rr) * ns] = v[b * RL + rr];
-    }
+    pass_tile<LOGR, SH::LOGCB, INV, FIRST>(in + g * N, out + g * N, q, logn, logns, pyr, fine, lds, threadIdx.x);
 }
 
 // ---- real pack / unpack for sizes above the one-workgroup limit ----------------------

@@ -35,6 +35,64 @@ int CkFftRealForwardBatch(CkFftContext* context, int n, int count,
 int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
                           const CkFftComplex* input, float* output, void* stream);
 
+/* ---- uneven batches in one persistent launch ------------------------------------------
+ *
+ * A batch plan takes any mix of transforms -- each with its own kind, size and device
+ * buffers -- and runs them as one launch of a persistent kernel (workgroups pull work
+ * items from an on-device queue, largest first).  Transforms the persistent kernel does
+ * not take (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches
+ * after it on the same stream; CkFftBatchInfo reports how many.
+ * Results are bit-identical to the single-signal calls for every transform.
+ * Executions of one plan must not overlap (they share its queue).  The reference has
+ * no counterpart for these entry points. */
+
+enum {
+    kCkFftKind_ComplexForward = 0,
+    kCkFftKind_ComplexInverse = 1,
+    kCkFftKind_RealForward = 2,
+    kCkFftKind_RealInverse = 3
+};
+
+typedef struct {
+    int kind;             /* kCkFftKind_* */
+    int n;                /* transform size, a power of two <= the context's nMax */
+    const void* input;    /* device memory, layout of the single-signal call */
+    void* output;         /* device memory */
+} CkFftBatchDesc;
+
+typedef struct {
+    int transforms;             /* descriptors in the plan */
+    int persistent_transforms;  /* handled by the persistent launch */
+    int work_items;             /* queue entries */
+    int workgroups;             /* persistent grid size */
+    int extra_launches;         /* transforms run as ordinary launches */
+    long long algorithmic_bytes;/* one read of every input, one write of every output */
+} CkFftBatchInfo;
+
+typedef struct _CkFftBatchPlan CkFftBatchPlan;
+
+/* Flag: capture the persistent launch in a hipGraph once and replay it. */
+#define kCkFftBatch_Graph 1
+
+/* Validate the descriptors (same rules as the single-signal calls, plus device
+ * pointers) and build the queue.  NULL on invalid arguments or allocation failure
+ * (CkFftBatchLastError tells which). */
+CkFftBatchPlan* CkFftBatchPlanCreate(CkFftContext* context, const CkFftBatchDesc* descs, int count, int flags);
+/* Enqueue one execution on `stream` (a hipStream_t; NULL = null stream).  1/0. */
+int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream);
+int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info);
+/* After synchronising: 1 if no execution flagged an internal timeout, else 0. */
+int CkFftBatchPlanCheck(CkFftBatchPlan* plan);
+void CkFftBatchPlanDestroy(CkFftBatchPlan* plan);
+const char* CkFftBatchLastError(void);
+
+/* Time both thread layouts of every class (kind, 32 <= n <= 8192 complex points) on
+ * this GPU, keep the faster in the context (used by later plans) and, if cachePath is
+ * not NULL, write them as JSON.  Layouts differ only in speed, never in results. */
+int CkFftAutotune(CkFftContext* context, const char* cachePath);
+/* Load choices written by CkFftAutotune.  1 if any entry was applied. */
+int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
+
 /* Copy the device twiddle table of level L (L values exp(-2 pi i k / L), k < L,
  * generated in FP64 on the GPU and rounded to FP32) to host memory `out`.
  * L must be a power of two no larger than the context's table.  Returns 1/0. */

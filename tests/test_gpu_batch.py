@@ -1,0 +1,196 @@
+"""GPU: uneven batches in one persistent launch (CkFftBatchPlan*) and the autotuner.
+
+Every transform of a plan must equal, bit for bit, the same transform run alone through
+the single-signal drop-in call (same arithmetic by construction), and match the CPU
+oracle within the parity gates of test_gpu_parity.py.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ("cfwd", "cinv", "rfwd", "rinv")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a HIP device"
+    return _t
+
+
+@pytest.fixture(scope="module")
+def C(torch):
+    import ckfft_amd
+    return ckfft_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(torch, C):
+    c = C.Context(1 << 17, C.kCkFftDirection_Both)
+    yield c
+    c.close()
+
+
+def in_len(kind, n):
+    return {"cfwd": 2 * n, "cinv": 2 * n, "rfwd": n, "rinv": 2 * (n // 2 + 1)}[kind]
+
+
+def out_len(kind, n):
+    return {"cfwd": 2 * n, "cinv": 2 * n, "rfwd": 2 * (n // 2 + 1), "rinv": n}[kind]
+
+
+def make_input(kind, n, seed):
+    if kind in ("cfwd", "cinv"):
+        return O.uniform_pm1(seed, 2 * n)
+    if kind == "rfwd":
+        return O.normal01(seed, n)
+    return O.Restatement(n, 3).real_forward(O.normal01(seed, n))
+
+
+def plain(torch, ctx, kind, n, xt):
+    y = torch.full((out_len(kind, n),), float("nan"), dtype=torch.float32, device="cuda")
+    if kind == "cfwd":
+        ok = ctx.complex_forward(n, xt, y)
+    elif kind == "cinv":
+        ok = ctx.complex_inverse(n, xt, y)
+    elif kind == "rfwd":
+        ok = ctx.real_forward(n, xt, y)
+    else:
+        ok = ctx.real_inverse(n, xt, y, torch.empty(2 * (n // 2 + 1), device="cuda"))
+    assert ok == 1
+    return y
+
+
+def random_mix(count, seed, kmin=0, kmax=17):
+    rng = np.random.default_rng(seed)
+    return [(KINDS[rng.integers(0, 4)], 1 << int(rng.integers(kmin, kmax + 1))) for _ in range(count)]
+
+
+def build(torch, C, ctx, mix, seed0):
+    ins, outs, descs = [], [], []
+    for i, (kind, n) in enumerate(mix):
+        x = torch.from_numpy(np.ascontiguousarray(make_input(kind, n, seed0 + i), dtype=np.float32)).cuda()
+        y = torch.full((out_len(kind, n),), float("nan"), dtype=torch.float32, device="cuda")
+        ins.append(x)
+        outs.append(y)
+        descs.append((KINDS.index(kind), n, x, y))
+    return ins, outs, descs
+
+
+def test_uneven_batch_equals_single_calls(torch, C, ctx):
+    mix = random_mix(160, 11)
+    mix += [("cfwd", 1 << 16), ("cinv", 1 << 16), ("rfwd", 1 << 17), ("cfwd", 1 << 17), ("rinv", 4), ("cfwd", 1)]
+    ins, outs, descs = build(torch, C, ctx, mix, 1000)
+    plan = C.BatchPlan(ctx, descs)
+    info = plan.info()
+    assert info["transforms"] == len(mix)
+    assert info["extra_launches"] >= 3          # the closed-form sizes and 2^17
+    assert plan.execute() == 1
+    torch.cuda.synchronize()
+    assert plan.check() == 1
+    for i, (kind, n) in enumerate(mix):
+        want = plain(torch, ctx, kind, n, ins[i]).cpu().numpy()
+        got = outs[i].cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (i, kind, n)
+    plan.close()
+
+
+def test_uneven_batch_against_oracle(torch, C, ctx):
+    mix = random_mix(60, 12, kmin=3, kmax=16)
+    ins, outs, descs = build(torch, C, ctx, mix, 2000)
+    plan = C.BatchPlan(ctx, descs)
+    assert plan.execute() == 1
+    torch.cuda.synchronize()
+    for i, (kind, n) in enumerate(mix):
+        x = ins[i].cpu().numpy()[None]
+        o = O.Restatement(n, 3)
+        ref = {"cfwd": lambda: o.complex(x, False), "cinv": lambda: o.complex(x, True),
+               "rfwd": lambda: o.real_forward(x), "rinv": lambda: o.real_inverse(x, n)}[kind]()
+        got = outs[i].cpu().numpy()[None]
+        assert O.split_norm_err(got, ref) <= 1e-5, (i, kind, n)
+
+
+def test_graph_replays_follow_inputs(torch, C, ctx):
+    """A graph-captured plan replays correctly with eager work between replays, and each
+    replay reads the inputs as they are at replay time."""
+    mix = random_mix(40, 13, kmin=4, kmax=16)
+    ins, outs, descs = build(torch, C, ctx, mix, 3000)
+    plan = C.BatchPlan(ctx, descs, graph=True)
+    for rep in range(3):
+        for i, (kind, n) in enumerate(mix):
+            if kind in ("cfwd", "cinv", "rfwd"):
+                ins[i].mul_(1.5)                 # eager work that changes the inputs
+        assert plan.execute() == 1
+        torch.cuda.synchronize()
+        for i, (kind, n) in enumerate(mix):
+            want = plain(torch, ctx, kind, n, ins[i]).cpu().numpy()
+            assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (rep, i, kind, n)
+    assert plan.check() == 1
+
+
+def test_autotune_changes_speed_not_results(torch, C, tmp_path):
+    with C.Context(1 << 16) as c:
+        path = str(tmp_path / "tune.json")
+        assert c.autotune(path) == 1
+        data = json.load(open(path))
+        assert data["arch"] == "gfx950" and len(data["choice"]) == 4 * 9
+        mix = random_mix(80, 14, kmin=5, kmax=14)
+        ins, outs, descs = build(torch, C, c, mix, 4000)
+        plan = C.BatchPlan(c, descs)
+        assert plan.execute() == 1
+        torch.cuda.synchronize()
+        for i, (kind, n) in enumerate(mix):
+            want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
+            assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
+        with C.Context(1 << 16) as c2:
+            assert c2.load_tuning(path) == 1
+
+
+def test_batch_plan_validation(torch, C, ctx):
+    x = torch.zeros(64, device="cuda")
+    y = torch.zeros(64, device="cuda")
+    h = np.zeros(64, np.float32)
+    bad = [
+        [(0, 16, x, x)],                 # in == out
+        [(0, 12, x, y)],                 # not a power of two
+        [(0, 1 << 18, x, y)],            # above nMax
+        [(5, 16, x, y)],                 # bad kind
+        [(0, 16, h, y)],                 # host input
+    ]
+    for descs in bad:
+        with pytest.raises(ValueError):
+            C.BatchPlan(ctx, descs)
+    with C.Context(64, C.kCkFftDirection_Forward) as f:
+        with pytest.raises(ValueError):
+            C.BatchPlan(f, [(1, 16, x, y)])  # inverse on a forward-only context
+    p = C.BatchPlan(ctx, [])
+    assert p.execute() == 1
+
+
+def test_config_uneven_sample_2000(torch, C, ctx):
+    """A 2000-transform sample of the uneven workload (k in 6..16, kinds and directions
+    50/50, N(0,1) data): every output against the single calls, bitwise."""
+    rng = np.random.default_rng(0xC4)
+    mix = []
+    for _ in range(2000):
+        k = int(rng.integers(6, 17))
+        real = rng.random() < 0.5
+        fwd = rng.random() < 0.5
+        kind = ("rfwd" if fwd else "rinv") if real else ("cfwd" if fwd else "cinv")
+        mix.append((kind, 1 << k))
+    ins, outs, descs = build(torch, C, ctx, mix, 5000)
+    plan = C.BatchPlan(ctx, descs, graph=True)
+    assert plan.info()["extra_launches"] == 0
+    assert plan.execute() == 1
+    torch.cuda.synchronize()
+    assert plan.check() == 1
+    for i in range(0, 2000, 7):
+        kind, n = mix[i]
+        want = plain(torch, ctx, kind, n, ins[i]).cpu().numpy()
+        assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
