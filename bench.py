@@ -7,6 +7,8 @@ in HBM before timing starts.  The value is whole-step throughput in GFLOP/s with
 events on the launch stream, which gives the roofline fraction against 8 TB/s.
 
 Also reported on the same line:
+  * batched_uneven: 20,000 transforms of mixed sizes (2^6..2^16), kinds and directions,
+    each with its own buffers, as one persistent launch (autotuned, graph-replayed);
   * plain: the same signals through the one-signal-per-call drop-in entry points
     (CkFftRealForward / CkFftRealInverse with device pointers, synchronous);
   * cpu_baseline: the reference's CPU path (oracle/_ref, built from the reference's
@@ -117,6 +119,86 @@ def cpu_baseline(n: int, seconds: float = 10.0):
             "cpu": cpu}
 
 
+def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 2000):
+    """20,000 transforms, n = 2^k with k uniform in 6..16, kinds 50% complex / 50% real and
+    directions 50/50, each with its own buffers, run as one persistent launch (autotuned,
+    replayed as a hipGraph).  FLOPs: 5 n log2 n complex, 2.5 n log2 n real."""
+    rng = np.random.default_rng(0xC4)
+    count = 20000
+    ks = rng.integers(6, 17, size=count)
+    real = rng.random(count) < 0.5
+    fwd = rng.random(count) < 0.5
+    kinds = np.where(real, np.where(fwd, 2, 3), np.where(fwd, 0, 1))
+    ns = (1 << ks).astype(np.int64)
+    in_len = np.where(kinds == 2, ns, np.where(kinds == 3, 2 * (ns // 2 + 1), 2 * ns))
+    out_len = np.where(kinds == 2, 2 * (ns // 2 + 1), np.where(kinds == 3, ns, 2 * ns))
+    align = 64                                             # 256-byte aligned buffers
+    in_off = np.concatenate([[0], np.cumsum((in_len + align - 1) // align * align)])
+    out_off = np.concatenate([[0], np.cumsum((out_len + align - 1) // align * align)])
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0xC4)
+    x = torch.randn(int(in_off[-1]), device=dev, generator=gen)
+    y = torch.empty(int(out_off[-1]), device=dev)
+    xb, yb = x.data_ptr(), y.data_ptr()
+    descs = [(int(kinds[i]), int(ns[i]), xb + 4 * int(in_off[i]), yb + 4 * int(out_off[i])) for i in range(count)]
+    flops = float(np.sum(np.where(real, 2.5, 5.0) * ns * ks))
+    ctx = C.Context(1 << 16, C.kCkFftDirection_Both)
+    t0 = time.perf_counter()
+    tuned = ctx.autotune(None) == 1
+    tune_s = time.perf_counter() - t0
+    plan = C.BatchPlan(ctx, descs, graph=True)
+    info = plan.info()
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(max(1, warmup)):
+        assert plan.execute(stream) == 1
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.execute(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gbs = info["algorithmic_bytes"] / (ms * 1e-3) / 1e9
+    # parity sample: a few transforms against the CPU oracle
+    worst = 0.0
+    for i in range(0, count, count // 12):
+        kind, n = int(kinds[i]), int(ns[i])
+        xi = x[int(in_off[i]): int(in_off[i]) + int(in_len[i])].cpu().numpy()[None]
+        yi = y[int(out_off[i]): int(out_off[i]) + int(out_len[i])].cpu().numpy()[None]
+        o = O.Restatement(n, 3)
+        ref = (o.complex(xi, kind == 1) if kind < 2 else o.real_forward(xi) if kind == 2 else o.real_inverse(xi, n))
+        worst = max(worst, O.split_norm_err(yi, ref))
+    # plain: the first `plain_sample` transforms, one synchronous drop-in call each
+    tmp = torch.empty(2 * ((1 << 16) // 2 + 1), device=dev)
+    calls = {0: ctx.complex_forward, 1: ctx.complex_inverse, 2: ctx.real_forward}
+    t0 = time.perf_counter()
+    pf = 0.0
+    for i in range(plain_sample):
+        kind, n = int(kinds[i]), int(ns[i])
+        xi, yi = xb + 4 * int(in_off[i]), yb + 4 * int(out_off[i])
+        ok = calls[kind](n, xi, yi) if kind < 3 else ctx.real_inverse(n, xi, yi, tmp)
+        assert ok == 1
+        pf += (2.5 if kind >= 2 else 5.0) * n * int(ks[i])
+    plain_s = time.perf_counter() - t0
+    plan.close()
+    ctx.close()
+    del x, y
+    return {
+        "workload": "20000 uneven transforms, n=2^k k~U{6..16}, 50% complex/real, 50% fwd/inv, own buffers",
+        "ms_per_batch": round(ms, 4), "value": round(flops / (ms * 1e-3) / 1e9, 1), "unit": "GFLOP/s",
+        "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes": info["algorithmic_bytes"], "launch": "one persistent kernel, hipGraph replay",
+        "work_items": info["work_items"], "workgroups": info["workgroups"],
+        "extra_launches": info["extra_launches"], "autotuned": tuned, "autotune_s": round(tune_s, 2),
+        "parity_worst_vs_ref": float(f"{worst:.3e}"),
+        "plain": {"value": round(pf / plain_s / 1e9, 2), "unit": "GFLOP/s",
+                  "sample": f"first {plain_sample} transforms, one synchronous call each"},
+        "batched_over_plain": round(flops / (ms * 1e-3) / (pf / plain_s), 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +206,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-signal leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (persistent launch) leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
                          "HBM counters FETCH_SIZE / WRITE_SIZE")
@@ -240,6 +323,8 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(n)
     ctx.close()
+    del x, y, z
+    uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=2)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
@@ -253,6 +338,7 @@ def main():
         "cpu_baseline": cpu,
         "plain": plain,
         "parity": parity,
+        "batched_uneven": uneven,
         "device": torch.cuda.get_device_name(dev),
     }
     print(json.dumps(line))

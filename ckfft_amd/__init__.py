@@ -21,7 +21,7 @@ __all__ = [
     "CkFftRealForward", "CkFftRealInverse", "CkFftComplexForwardBatch", "CkFftComplexInverseBatch",
     "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "CkFftGpuLastError", "Context", "BatchPlan", "CkFftBatchDesc",
     "CkFftBatchInfo", "CkFftBatchPlanCreate", "CkFftBatchPlanExecute", "CkFftBatchPlanGetInfo",
-    "CkFftBatchPlanCheck", "CkFftBatchPlanDestroy", "CkFftBatchLastError", "CkFftAutotune", "CkFftLoadTuning",
+    "CkFftBatchPlanDestroy", "CkFftBatchLastError", "CkFftAutotune", "CkFftLoadTuning",
     "EXPORTED_SYMBOLS",
 ]
 
@@ -90,7 +90,6 @@ _SIGS = {
     "CkFftBatchPlanCreate": (_vp, [_vp, ctypes.POINTER(CkFftBatchDesc), _i, _i]),
     "CkFftBatchPlanExecute": (_i, [_vp, _vp]),
     "CkFftBatchPlanGetInfo": (_i, [_vp, ctypes.POINTER(CkFftBatchInfo)]),
-    "CkFftBatchPlanCheck": (_i, [_vp]),
     "CkFftBatchPlanDestroy": (None, [_vp]),
     "CkFftBatchLastError": (ctypes.c_char_p, []),
     "CkFftAutotune": (_i, [_vp, ctypes.c_char_p]),
@@ -208,9 +207,6 @@ class BatchPlan:
 
     def execute(self, stream=None) -> int:
         return CkFftBatchPlanExecute(self.handle, _stream(stream))
-
-    def check(self) -> int:
-        return CkFftBatchPlanCheck(self.handle)
 
     def info(self) -> dict:
         inf = CkFftBatchInfo()

@@ -5,16 +5,16 @@
 // mix of sizes, kinds and directions (each with its own device buffers) runs as ONE
 // launch of a persistent kernel:
 //  * the host groups the transforms into work items -- a group of same-class transforms
-//    that fills one 512-thread workgroup, or one tile of a multi-pass transform -- and
-//    orders them largest-first (first-pass tiles first, so that no dequeued tile can
-//    wait on work nobody has taken);
+//    that fills one 512-thread workgroup, or one whole complex 2^16 transform -- and
+//    orders them largest-first;
 //  * one workgroup per CU loops: take the next item from an atomic queue, run it with
 //    the same transform bodies the batch kernels use (fft_lds.h, fft_pass.h);
-//  * a second-pass tile waits (bounded) until the transform's first-pass tiles have
-//    published their results (agent-scope release/acquire), so a complex 2^16 transform
-//    splits into 16 independent tiles;
-//  * the last workgroup to leave resets the queue and counters, so a plan re-executes
-//    (eagerly or as a replayed hipGraph) with no other launch.
+//  * a complex 2^16 transform runs as its two passes inside one workgroup: 8 first-pass
+//    tiles into a scratch buffer, a block barrier (which makes the block's global writes
+//    visible to the block), 8 second-pass tiles.  Items never wait on each other, so
+//    the kernel needs no cross-workgroup synchronisation and no cache flushes;
+//  * the last workgroup to leave resets the queue, so a plan re-executes (eagerly or as
+//    a replayed hipGraph) with no other launch.
 // Transforms the persistent kernel does not take (closed-form sizes, sizes above 2^16)
 // run through the ordinary launches after it, on the same stream.
 //
@@ -38,28 +38,25 @@
 namespace ckfft_gpu {
 
 struct BatchItem {
-    int cls;        // class id (lds_cls / kClsPass + ...)
+    int cls;        // class id (lds_cls, or kClsPass + inverse)
     int first;      // first transform (index into the BatchTransform table)
     int count;      // transforms in this item (single-workgroup classes)
-    int tile;       // tile index (pass classes)
+    int pad;
 };
 
 struct BatchTransform {
     const float2* in;
     float2* out;
-    float2* scratch;   // intermediate of a multi-pass transform
-    int dep;           // its dependency counter
-    int pad;
+    float2* scratch;   // intermediate of a multi-pass transform (else null)
 };
 
 constexpr int kPersistWG = 512;
 constexpr int kPersistLdsFloats = 34816;        // 139264 bytes
-constexpr int kClsPass = 4096;                  // + 2*inverse + first
+constexpr int kClsPass = 4096;                  // + inverse
 constexpr int kPassLogN = 16, kPassLogR = 8, kPassLogCB = 5;
 constexpr int kPassTiles = (1 << (kPassLogN - kPassLogR)) >> kPassLogCB;
-constexpr int kSpinLimit = 1 << 22;
-// control words: [0] queue head, [1] error flags, [2] workgroups finished, [3..] deps
-constexpr int kCtlHeader = 3;
+// control words: [0] queue head, [1] workgroups finished
+constexpr int kCtlWords = 2;
 
 __host__ __device__ constexpr int persist_logp(int logc, int var)
 {
@@ -81,6 +78,7 @@ __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTran
     // and keeps it live through every body, which spills the large classes.
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < kPersistWG);
     const int slot = tid / SH::T;
     const bool active = slot < w.count;
     const BatchTransform d = trs[w.first + (active ? slot : 0)];
@@ -112,58 +110,46 @@ __device__ __forceinline__ void dispatch_lds(int logc, int var, const BatchItem&
     (void)((logc == LO + L ? run_lds_var<KIND, LO + L>(var, w, trs, tw, lds) : 0), ...);
 }
 
-template <bool INV, bool FIRST>
-__device__ __forceinline__ void run_pass_item(const BatchItem& w, const BatchTransform* __restrict__ trs, int* ctl,
-                                              const DeviceTables& tb, float* lds)
+// A complex 2^16 transform: both passes (radix 2^8 each, the same radices as the
+// stand-alone multi-pass path) in this workgroup, tile after tile.
+template <bool INV>
+__device__ __forceinline__ void run_twopass_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
+                                                 const DeviceTables& tb, float* lds)
 {
     const BatchTransform d = trs[w.first];
-    int* dep = ctl + kCtlHeader + d.dep;
-    if constexpr (!FIRST) {
-        if (threadIdx.x == 0) {
-            int spins = 0;
-            while (__hip_atomic_load(dep, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < kPassTiles) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > kSpinLimit) {          // never expected: flag and move on
-                    atomicOr(ctl + 1, 1);
-                    break;
-                }
-            }
-        }
+    for (int q = 0; q < kPassTiles; ++q) {
+        pass_tile<kPassLogR, kPassLogCB, INV, true>(d.in, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, lds,
+                                                     threadIdx.x);
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    pass_tile<kPassLogR, kPassLogCB, INV, FIRST>(FIRST ? d.in : d.scratch, FIRST ? d.scratch : d.out, w.tile,
-                                                  kPassLogN, FIRST ? 0 : kPassLogR, tb.pyramid, tb.fine, lds,
-                                                  threadIdx.x);
-    if constexpr (FIRST) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int q = 0; q < kPassTiles; ++q) {
+        pass_tile<kPassLogR, kPassLogCB, INV, false>(d.scratch, d.out, q, kPassLogN, kPassLogR, tb.pyramid, tb.fine,
+                                                      lds, threadIdx.x);
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_fetch_add(dep, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchItem* __restrict__ items, int n_items,
                                                                      const BatchTransform* __restrict__ trs,
-                                                                     int* __restrict__ ctl, int n_deps,
-                                                                     DeviceTables tb)
+                                                                     int* __restrict__ ctl, DeviceTables tb)
 {
     __shared__ float lds[kPersistLdsFloats];
     __shared__ int s_item;
     const int tid = threadIdx.x;
+    if (tid == 0) s_item = atomicAdd(ctl, 1);
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(ctl, 1);
         __syncthreads();
         const int it = s_item;
         __syncthreads();
         if (it >= n_items) break;
+        // claim the next item now, so the queue round trip overlaps this item's work
+        if (tid == 0) s_item = atomicAdd(ctl, 1);
         const BatchItem w = items[it];
         if (w.cls >= kClsPass) {
-            switch (w.cls - kClsPass) {
-            case 0: run_pass_item<false, false>(w, trs, ctl, tb, lds); break;
-            case 1: run_pass_item<false, true>(w, trs, ctl, tb, lds); break;
-            case 2: run_pass_item<true, false>(w, trs, ctl, tb, lds); break;
-            default: run_pass_item<true, true>(w, trs, ctl, tb, lds); break;
-            }
+            if (w.cls == kClsPass)
+                run_twopass_item<false>(w, trs, tb, lds);
+            else
+                run_twopass_item<true>(w, trs, tb, lds);
         } else {
             const int var = w.cls & 1;
             const int kind = (w.cls >> 1) >> 4;
@@ -185,14 +171,13 @@ __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchI
         }
         __syncthreads();   // the LDS buffer is reused by the next item
     }
-    // the last workgroup out resets the control words for the next execution
+    // the last workgroup out resets the queue for the next execution (every other
+    // workgroup has left the loop, so none reads the head again in this launch)
     if (tid == 0) {
-        const int prev = atomicAdd(ctl + 2, 1);
+        const int prev = atomicAdd(ctl + 1, 1);
         if (prev == (int)gridDim.x - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            for (int i = 0; i < n_deps; ++i) ctl[kCtlHeader + i] = 0;
-            ctl[0] = 0;
-            ctl[2] = 0;
+            atomicExch(ctl, 0);
+            atomicExch(ctl + 1, 0);
         }
     }
 }
@@ -246,7 +231,6 @@ struct _CkFftBatchPlan {
     CkFftContext* ctx;
     int n_items = 0;
     int n_trs = 0;
-    int n_deps = 0;
     int grid = 0;
     BatchItem* d_items = nullptr;
     BatchTransform* d_trs = nullptr;
@@ -310,25 +294,22 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
         });
         std::vector<BatchTransform> trs(pers.size());
         std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item)
-        std::vector<BatchItem> first_pass;
-        int n_deps = 0;
+        size_t n_two = 0;
         for (size_t i = 0; i < pers.size(); ++i) {
             const CkFftBatchDesc& d = descs[pers[i].idx];
-            trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr, -1, 0};
-            if (pers[i].logc == kPassLogN) trs[i].dep = n_deps++;
+            trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr};
+            if (pers[i].logc == kPassLogN) ++n_two;
         }
-        const size_t scratch_pts = (size_t)n_deps << kPassLogN;
-        if (n_deps && !berr(hipMalloc(&plan->d_scratch, sizeof(float2) * scratch_pts), "hipMalloc(batch scratch)"))
+        if (n_two && !berr(hipMalloc(&plan->d_scratch, sizeof(float2) * (n_two << kPassLogN)), "hipMalloc(batch scratch)"))
             goto bad;
+        n_two = 0;
         for (size_t i = 0; i < pers.size();) {
             const Entry& e = pers[i];
             if (e.logc == kPassLogN) {
-                trs[i].scratch = plan->d_scratch + ((size_t)trs[i].dep << kPassLogN);
+                trs[i].scratch = plan->d_scratch + (n_two++ << kPassLogN);
                 const int inv = e.kind == kComplexInverse ? 1 : 0;
-                for (int q = 0; q < kPassTiles; ++q) {
-                    first_pass.push_back(BatchItem{kClsPass + 2 * inv + 1, (int)i, 1, q});
-                    work.push_back({2ll * (8 << kPassLogR) << kPassLogCB, BatchItem{kClsPass + 2 * inv, (int)i, 1, q}});
-                }
+                // both passes move the signal: 4 x 16 bytes per point
+                work.push_back({64ll << kPassLogN, BatchItem{kClsPass + inv, (int)i, 1, 0}});
                 ++i;
                 continue;
             }
@@ -346,20 +327,19 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
                          [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
                              return a.first > b.first;
                          });
-        std::vector<BatchItem> items = first_pass;
+        std::vector<BatchItem> items;
         for (auto& w : work) items.push_back(w.second);
         plan->n_items = (int)items.size();
         plan->n_trs = (int)trs.size();
-        plan->n_deps = n_deps;
         if (plan->n_items) {
             if (!berr(hipMalloc(&plan->d_items, sizeof(BatchItem) * items.size()), "hipMalloc(items)") ||
                 !berr(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * trs.size()), "hipMalloc(transforms)") ||
-                !berr(hipMalloc(&plan->d_ctl, sizeof(int) * (kCtlHeader + n_deps)), "hipMalloc(control)") ||
+                !berr(hipMalloc(&plan->d_ctl, sizeof(int) * kCtlWords), "hipMalloc(control)") ||
                 !berr(hipMemcpy(plan->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice),
                       "upload items") ||
                 !berr(hipMemcpy(plan->d_trs, trs.data(), sizeof(BatchTransform) * trs.size(), hipMemcpyHostToDevice),
                       "upload transforms") ||
-                !berr(hipMemset(plan->d_ctl, 0, sizeof(int) * (kCtlHeader + n_deps)), "clear control"))
+                !berr(hipMemset(plan->d_ctl, 0, sizeof(int) * kCtlWords), "clear control"))
                 goto bad;
             int per_cu = 0, cus = 0;
             if (!berr(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_persistent_kernel, kPersistWG, 0),
@@ -388,7 +368,7 @@ hipError_t launch_persistent(CkFftBatchPlan* p, hipStream_t s)
     const CkFftContext* c = p->ctx;
     DeviceTables tb{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
     hipLaunchKernelGGL(fft_persistent_kernel, dim3(p->grid), dim3(kPersistWG), 0, s, p->d_items, p->n_items, p->d_trs,
-                       p->d_ctl, p->n_deps, tb);
+                       p->d_ctl, tb);
     return hipGetLastError();
 }
 
@@ -434,21 +414,6 @@ int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info)
 {
     if (!plan || !info) return 0;
     *info = plan->info;
-    return 1;
-}
-
-int CkFftBatchPlanCheck(CkFftBatchPlan* plan)
-{
-    if (!plan) return 0;
-    if (!plan->d_ctl) return 1;
-    int flags = 0;
-    if (!berr(hipMemcpy(&flags, plan->d_ctl + 1, sizeof(int), hipMemcpyDeviceToHost), "read flags")) return 0;
-    if (flags) {
-        const int zero = 0;
-        (void)hipMemcpy(plan->d_ctl + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
-        std::snprintf(g_batch_error, sizeof(g_batch_error), "persistent kernel flagged a dependency timeout");
-        return 0;
-    }
     return 1;
 }
 

@@ -37,16 +37,19 @@ namespace ckfft_gpu {
 
 // Which stage (if any) uses the mirror-closed assignment: the first stage of a real
 // inverse (unpack fused into its loads) and the last stage of a real forward (pack fused
-// into its stores), when that stage has an even number >= 2 of butterflies per thread.
+// into its stores), when that edge stage's radix is at most 8.  The rule looks only at
+// the radix plan, never at the thread layout: every layout of a size (any points per
+// thread >= 16) then has >= 2 butterflies per thread there and takes the same path, so
+// a transform rounds identically in every kernel that runs it.
 template <class PL, int KIND>
 constexpr int mirror_stage()
 {
-    if constexpr (KIND == kRealInverse) {
-        constexpr int B = bflies<PL, 0>();
-        return (B >= 2 && B % 2 == 0) ? 0 : -1;
+    if constexpr (PL::T == 1) {
+        return -1;
+    } else if constexpr (KIND == kRealInverse) {
+        return (PL::rlog(0) <= 3 && bflies<PL, 0>() >= 2) ? 0 : -1;
     } else if constexpr (KIND == kRealForward) {
-        constexpr int B = bflies<PL, PL::NSTAGES - 1>();
-        return (B >= 2 && B % 2 == 0) ? PL::NSTAGES - 1 : -1;
+        return (PL::rlog(PL::NSTAGES - 1) <= 3 && bflies<PL, PL::NSTAGES - 1>() >= 2) ? PL::NSTAGES - 1 : -1;
     } else {
         return -1;
     }
@@ -68,8 +71,10 @@ __device__ __forceinline__ void run_stages(float2* v, int t, float* re, float* i
 // exp(-2 pi i k / 2N) for point k (0 <= k <= N) of the real pack/unpack of an N-point half
 // transform whose edge stage (the one fused with the pack or unpack) has radix R: with
 // k = j + r*M, M = N/R, it is the table value W^j times the constant exp(-2 pi i r/2R).
-// Every code path forms it this way, so a transform's result does not depend on which
-// thread mapping or kernel computed it.
+// The non-mirror paths form every twiddle this way; the mirror-closed paths form the
+// lower half this way and take each upper point's as -conj of its partner's.  Which of
+// the two a size uses depends only on its radix plan (mirror_stage), so every kernel
+// computing a given size rounds the same way.
 template <int N, int R>
 __device__ __forceinline__ float2 edge_twiddle(const float2* __restrict__ lvl, int k)
 {
@@ -151,18 +156,21 @@ __device__ __forceinline__ void load_first_unpack_mirror(float2* v, int t, const
     for (int b = 0; b < H; ++b) {
         const int jl = t + b * PL::T;
         const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wl = lvl[jl], wh = lvl[jh];         // edge_twiddle's table factors
+        const float2 wl = lvl[jl];                       // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             float2& lo = v[b * R + r];
             float2& hi = v[(b + H) * R + R - 1 - r];
             const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
             const float2 w0 = cmul(wl, w64(r * (32 / R)));
-            const float2 w1 = cmul(wh, w64((R - 1 - r) * (32 / R)));
             float2 p0 = hi, p1 = lo;                     // Y[N - k0], Y[N - k1] when paired
-            if (b == 0 && t == 0) {                      // butterflies 0 and M/2 of thread 0
+            float2 w1 = make_float2(-w0.x, w0.y);        // W^(N-k0) = -conj(W^k0)
+            if (b == 0 && t == 0) {
+                // butterflies 0 and M/2 of thread 0: partners from memory, and
+                // W^(M/2 + (R-1-r)M) = exp(-2 pi i (2(R-1-r)+1) / 4R)
                 p0 = y[N - k0];
                 p1 = y[N - k1];
+                w1 = w64((2 * (R - 1 - r) + 1) * (16 / R));
             }
             const float2 y0 = lo, y1 = hi;
             lo = unpack_one(y0, p0, w0);
@@ -227,20 +235,22 @@ __device__ __forceinline__ void pack_store_mirror(const float2* v, int t, const 
     for (int b = 0; b < H; ++b) {
         const int jl = t + b * PL::T;
         const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wl = lvl[jl], wh = lvl[jh];  // edge_twiddle's table factors
+        const float2 wl = lvl[jl];               // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float2 lo = v[b * R + r];
             const float2 hi = v[(b + H) * R + R - 1 - r];
             const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
             const float2 w0 = cmul(wl, w64(r * (32 / R)));
-            const float2 w1 = cmul(wh, w64((R - 1 - r) * (32 / R)));
             float2 p0 = hi, p1 = lo;
+            float2 w1 = make_float2(-w0.x, w0.y);   // W^(N-k0) = -conj(W^k0)
             if (b == 0) {
-                // thread 0: point r*M pairs with (R-r)*M, point M/2 + (R-1-r)*M with M/2 + r*M
+                // thread 0: point r*M pairs with (R-r)*M, point M/2 + (R-1-r)*M with M/2 + r*M,
+                // whose twiddle is exp(-2 pi i (2(R-1-r)+1) / 4R)
                 const bool z = (t == 0);
                 p0 = z ? v[(R - r) & (R - 1)] : p0;
                 p1 = z ? v[H * R + r] : p1;
+                w1 = z ? w64((2 * (R - 1 - r) + 1) * (16 / R)) : w1;
             }
             dst[k0] = pack_one(lo, p0, w0);
             dst[k1] = pack_one(hi, p1, w1);

@@ -81,8 +81,6 @@ CkFftBatchPlan* CkFftBatchPlanCreate(CkFftContext* context, const CkFftBatchDesc
 /* Enqueue one execution on `stream` (a hipStream_t; NULL = null stream).  1/0. */
 int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream);
 int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info);
-/* After synchronising: 1 if no execution flagged an internal timeout, else 0. */
-int CkFftBatchPlanCheck(CkFftBatchPlan* plan);
 void CkFftBatchPlanDestroy(CkFftBatchPlan* plan);
 const char* CkFftBatchLastError(void);
 
