@@ -14,7 +14,10 @@ Also reported on the same line:
   * cpu_baseline: the reference's CPU path (oracle/_ref, built from the reference's
     own sources; the CPU restatement if that is absent) timed on this host's cores on a
     bounded sample of the same workload;
-  * parity: a sample of the benchmarked outputs against the CPU oracle and FP64.
+  * parity: a sample of the benchmarked outputs against the CPU oracle and FP64;
+  * serving: Poisson arrivals at 200k requests/s of the uneven mix through the
+    continuous-batching server under a 4 GiB HBM budget: p50/p99 latency, throughput,
+    and a bitwise alone-vs-mix check on a sample of the outputs.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -119,12 +122,10 @@ def cpu_baseline(n: int, seconds: float = 10.0):
             "cpu": cpu}
 
 
-def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 2000):
-    """20,000 transforms, n = 2^k with k uniform in 6..16, kinds 50% complex / 50% real and
-    directions 50/50, each with its own buffers, run as one persistent launch (autotuned,
-    replayed as a hipGraph).  FLOPs: 5 n log2 n complex, 2.5 n log2 n real."""
-    rng = np.random.default_rng(0xC4)
-    count = 20000
+def uneven_mix(torch, dev, count: int, seed: int):
+    """`count` transforms, n = 2^k with k uniform in 6..16, kinds 50% complex / 50% real and
+    directions 50/50, iid N(0,1) inputs, each with its own 256-byte aligned buffers."""
+    rng = np.random.default_rng(seed)
     ks = rng.integers(6, 17, size=count)
     real = rng.random(count) < 0.5
     fwd = rng.random(count) < 0.5
@@ -136,12 +137,25 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
     in_off = np.concatenate([[0], np.cumsum((in_len + align - 1) // align * align)])
     out_off = np.concatenate([[0], np.cumsum((out_len + align - 1) // align * align)])
     gen = torch.Generator(device=dev)
-    gen.manual_seed(0xC4)
+    gen.manual_seed(seed)
     x = torch.randn(int(in_off[-1]), device=dev, generator=gen)
     y = torch.empty(int(out_off[-1]), device=dev)
     xb, yb = x.data_ptr(), y.data_ptr()
     descs = [(int(kinds[i]), int(ns[i]), xb + 4 * int(in_off[i]), yb + 4 * int(out_off[i])) for i in range(count)]
-    flops = float(np.sum(np.where(real, 2.5, 5.0) * ns * ks))
+    flops = np.where(real, 2.5, 5.0) * ns * ks
+    return dict(ks=ks, ns=ns, kinds=kinds, in_off=in_off, out_off=out_off, in_len=in_len, out_len=out_len,
+                x=x, y=y, descs=descs, flops=flops)
+
+
+def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 2000):
+    """20,000 transforms of the uneven mix run as one persistent launch (autotuned,
+    replayed as a hipGraph).  FLOPs: 5 n log2 n complex, 2.5 n log2 n real."""
+    count = 20000
+    m = uneven_mix(torch, dev, count, 0xC4)
+    ks, ns, kinds, x, y, descs = m["ks"], m["ns"], m["kinds"], m["x"], m["y"], m["descs"]
+    in_off, out_off, in_len, out_len = m["in_off"], m["out_off"], m["in_len"], m["out_len"]
+    xb, yb = x.data_ptr(), y.data_ptr()
+    flops = float(np.sum(m["flops"]))
     ctx = C.Context(1 << 16, C.kCkFftDirection_Both)
     t0 = time.perf_counter()
     tuned = ctx.autotune(None) == 1
@@ -199,6 +213,65 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
     }
 
 
+def serving_leg(torch, C, dev, rate: float = 200e3, seconds: float = 2.0, budget: int = 4 << 30,
+                slots: int = 4, pool: int = 20000):
+    """Poisson arrivals at `rate` requests/s for `seconds`, sizes and kinds drawn as in the
+    uneven mix, served by the continuous-batching server under an HBM admission budget.
+    Requests cycle over a pool of `pool` distinct buffer pairs; two requests sharing a pair
+    arrive pool/rate seconds apart (100 ms at the defaults), far beyond any latency here.
+    Latency is arrival (scheduled time) to completion seen by the dispatcher."""
+    m = uneven_mix(torch, dev, pool, 0x5E)
+    kinds, ns, x, y = m["kinds"], m["ns"], m["x"], m["y"]
+    torch.cuda.synchronize()
+    count = int(rate * seconds)
+    arrivals = np.cumsum(np.random.default_rng(0x5E).exponential(1e6 / rate, count))
+    descs = [m["descs"][i % pool] for i in range(count)]
+    offered_bytes = float(np.mean([(4 * n + 8 * (n // 2 + 1)) if k >= 2 else 16 * n
+                                   for k, n in zip(kinds, ns)]))
+    ctx = C.Context(1 << 16, C.kCkFftDirection_Both)
+    ctx.autotune(None)
+    out = {}
+    with C.Server(ctx, budget, max_in_flight=slots) as srv:
+        # unloaded: one request at a time (2,000 req/s), the latency floor
+        srv.replay(descs[:400], np.arange(400) * 500.0)
+        assert srv.drain() == 1
+        idle = srv.stats()
+        srv.reset_stats()
+        srv.replay(descs, arrivals)
+        assert srv.drain() == 1
+        st = srv.stats()
+    # alone vs in the mix: the last request on each sampled buffer pair against a single call
+    tmp = torch.empty(2 * ((1 << 16) // 2 + 1), device=dev)
+    calls = {0: ctx.complex_forward, 1: ctx.complex_inverse, 2: ctx.real_forward}
+    mismatches = 0
+    checked = 0
+    for j in range(0, pool, pool // 200):
+        kind, n = int(kinds[j]), int(ns[j])
+        xi = x[int(m["in_off"][j]): int(m["in_off"][j]) + int(m["in_len"][j])]
+        yi = y[int(m["out_off"][j]): int(m["out_off"][j]) + int(m["out_len"][j])]
+        want = torch.empty_like(yi)
+        ok = calls[kind](n, xi, want) if kind < 3 else ctx.real_inverse(n, xi, want, tmp)
+        assert ok == 1
+        mismatches += int(not torch.equal(want.view(torch.int32), yi.view(torch.int32)))
+        checked += 1
+    ctx.close()
+    del x, y
+    out = {
+        "workload": f"Poisson {rate:.0f} req/s for {seconds:g} s, uneven mix (n=2^k k~U{{6..16}}, "
+                    f"50% complex/real, 50% fwd/inv), HBM budget {budget / 2**30:g} GiB, {slots} slots",
+        "requests": count, "completed": st["completed"], "failed": st["failed"],
+        "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
+        "mean_us": round(st["mean_us"], 1),
+        "throughput_req_s": round(st["requests_per_s"], 0), "offered_req_s": rate,
+        "achieved_GBs": round(st["GB_per_s"], 2), "offered_GBs": round(rate * offered_bytes / 1e9, 2),
+        "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2),
+        "peak_inflight_bytes": st["peak_inflight_bytes"], "budget_bytes": st["budget_bytes"],
+        "unloaded": {"rate_req_s": 2000, "p50_us": round(idle["p50_us"], 1), "p99_us": round(idle["p99_us"], 1)},
+        "alone_vs_mix": {"checked": checked, "bitwise_mismatches": mismatches},
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +280,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-signal leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (persistent launch) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
                          "HBM counters FETCH_SIZE / WRITE_SIZE")
@@ -325,6 +399,7 @@ def main():
     ctx.close()
     del x, y, z
     uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=2)
+    serving = None if args.no_serve else serving_leg(torch, C, dev)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
@@ -339,6 +414,7 @@ def main():
         "plain": plain,
         "parity": parity,
         "batched_uneven": uneven,
+        "serving": serving,
         "device": torch.cuda.get_device_name(dev),
     }
     print(json.dumps(line))

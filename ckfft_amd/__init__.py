@@ -22,7 +22,9 @@ __all__ = [
     "CkFftRealForwardBatch", "CkFftRealInverseBatch", "CkFftGetTwiddles", "CkFftGpuVersion", "CkFftGpuLastError", "Context", "BatchPlan", "CkFftBatchDesc",
     "CkFftBatchInfo", "CkFftBatchPlanCreate", "CkFftBatchPlanExecute", "CkFftBatchPlanGetInfo",
     "CkFftBatchPlanDestroy", "CkFftBatchLastError", "CkFftAutotune", "CkFftLoadTuning",
-    "EXPORTED_SYMBOLS",
+    "CkFftServerStats", "CkFftServerCreate", "CkFftServerSubmit", "CkFftServerReplay", "CkFftServerQuery",
+    "CkFftServerWait", "CkFftServerDrain", "CkFftServerGetStats", "CkFftServerResetStats",
+    "CkFftServerDestroy", "CkFftServerLastError", "Server", "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
@@ -70,8 +72,19 @@ class CkFftBatchInfo(ctypes.Structure):
                 ("work_items", ctypes.c_int), ("workgroups", ctypes.c_int), ("extra_launches", ctypes.c_int),
                 ("algorithmic_bytes", ctypes.c_longlong)]
 
+
+class CkFftServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong), ("p50_us", ctypes.c_double),
+                ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double), ("mean_us", ctypes.c_double),
+                ("requests_per_s", ctypes.c_double), ("GB_per_s", ctypes.c_double),
+                ("peak_inflight_bytes", ctypes.c_longlong), ("budget_bytes", ctypes.c_longlong),
+                ("resident_bytes", ctypes.c_longlong), ("batches", ctypes.c_longlong),
+                ("mean_batch", ctypes.c_double)]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
+_ll = ctypes.c_longlong
 
 _SIGS = {
     "CkFftInit": (_vp, [_i, _i, _vp, ctypes.POINTER(ctypes.c_size_t)]),
@@ -94,6 +107,17 @@ _SIGS = {
     "CkFftBatchLastError": (ctypes.c_char_p, []),
     "CkFftAutotune": (_i, [_vp, ctypes.c_char_p]),
     "CkFftLoadTuning": (_i, [_vp, ctypes.c_char_p]),
+    "CkFftServerCreate": (_vp, [_vp, ctypes.c_size_t, _i]),
+    "CkFftServerSubmit": (_ll, [_vp, _i, _i, _vp, _vp]),
+    "CkFftServerReplay": (_i, [_vp, ctypes.POINTER(CkFftBatchDesc), ctypes.POINTER(ctypes.c_double), _i,
+                               ctypes.POINTER(_ll)]),
+    "CkFftServerQuery": (_i, [_vp, _ll]),
+    "CkFftServerWait": (_i, [_vp, _ll]),
+    "CkFftServerDrain": (_i, [_vp]),
+    "CkFftServerGetStats": (_i, [_vp, ctypes.POINTER(CkFftServerStats)]),
+    "CkFftServerResetStats": (None, [_vp]),
+    "CkFftServerDestroy": (None, [_vp]),
+    "CkFftServerLastError": (ctypes.c_char_p, []),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -217,6 +241,69 @@ class BatchPlan:
         if self.handle:
             CkFftBatchPlanDestroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching serving loop (CkFftServer*): requests admitted against an HBM
+    budget as they arrive, batched into persistent launches on up to ``max_in_flight``
+    streams at once.  Buffers are device memory the caller keeps alive until done."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_in_flight: int = 4):
+        self.ctx = ctx                         # keeps the context alive
+        self.handle = CkFftServerCreate(ctx.handle, int(hbm_budget), int(max_in_flight))
+        if not self.handle:
+            raise ValueError(f"CkFftServerCreate failed: {CkFftServerLastError().decode()}")
+
+    def submit(self, kind, n, x, y) -> int:
+        return CkFftServerSubmit(self.handle, kind, n, _ptr(x), _ptr(y))
+
+    def replay(self, transforms, arrival_us):
+        """Submit (kind, n, input, output) transforms at the given arrival times
+        (microseconds from the call); returns the list of tickets."""
+        count = len(transforms)
+        arr = (CkFftBatchDesc * max(1, count))()
+        for i, (kind, n, x, y) in enumerate(transforms):
+            arr[i] = CkFftBatchDesc(kind, n, _ptr(x), _ptr(y))
+        times = (ctypes.c_double * max(1, count))(*[float(t) for t in arrival_us])
+        tickets = (_ll * max(1, count))()
+        ok = CkFftServerReplay(self.handle, arr, times, count, tickets)
+        if not ok:
+            raise ValueError(f"CkFftServerReplay rejected a request: {CkFftServerLastError().decode()}")
+        return list(tickets)[:count]
+
+    def query(self, ticket) -> int:
+        return CkFftServerQuery(self.handle, ticket)
+
+    def wait(self, ticket) -> int:
+        return CkFftServerWait(self.handle, ticket)
+
+    def drain(self) -> int:
+        return CkFftServerDrain(self.handle)
+
+    def stats(self) -> dict:
+        st = CkFftServerStats()
+        assert CkFftServerGetStats(self.handle, ctypes.byref(st)) == 1
+        return {f: getattr(st, f) for f, _ in CkFftServerStats._fields_}
+
+    def reset_stats(self):
+        CkFftServerResetStats(self.handle)
+
+    def close(self):
+        if self.handle:
+            CkFftServerDestroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -1,0 +1,63 @@
+// fft_batch.h -- host-side interface of the persistent batch kernel, shared by batch
+// plans (fft_batch.hip) and the serving loop (fft_serve.cpp).
+#pragma once
+#include <vector>
+
+#include "ckfft/ckfft_gpu.h"
+#include "ckfft_internal.h"
+
+namespace ckfft_gpu {
+
+// One queue entry of the persistent kernel.
+struct BatchItem {
+    int cls;        // class id (see fft_batch.hip)
+    int first;      // first transform (index into the BatchTransform table)
+    int count;      // transforms in this item (single-workgroup classes)
+    int pad;
+};
+
+// The buffers of one transform taken by the persistent kernel.
+struct BatchTransform {
+    const float2* in;
+    float2* out;
+    float2* scratch;   // intermediate of a complex 2^16 transform (else null)
+};
+
+// A transform the persistent kernel does not take: run with an ordinary launch.
+struct BatchExtra {
+    Kind kind;
+    int log2n;
+    const void* in;
+    void* out;
+};
+
+constexpr int kPersistWG = 512;       // threads per persistent workgroup
+constexpr int kPassLogN = 16;         // complex size run as two passes inside one workgroup
+constexpr int kCtlWords = 2;          // control words of one launch: queue head, finished count
+
+// Host description of a batch.
+struct BatchBuild {
+    std::vector<BatchItem> items;
+    std::vector<BatchTransform> trs;  // scratch pointers set by batch_assign_scratch
+    std::vector<BatchExtra> extras;
+    std::vector<int> twopass;         // transforms needing (1 << kPassLogN) points of scratch
+    long long bytes = 0;              // algorithmic bytes of the whole batch
+    int persistent = 0;               // transforms taken by the persistent kernel
+};
+
+// Validation of one descriptor (single-signal rules, plus device buffers when asked);
+// false records the reason for CkFftBatchLastError.
+bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int index, bool check_pointers);
+// Validate and group a batch; var_override >= 0 forces one thread layout for every class.
+bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
+                 bool check_pointers, BatchBuild& out);
+void batch_assign_scratch(BatchBuild& b, float2* scratch);
+int persistent_grid(const CkFftContext* ctx);
+hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
+                                 int grid, const DeviceTables& tb, hipStream_t s);
+DeviceTables context_tables(const CkFftContext* c);
+long long transform_bytes(int kind, int log2n);
+bool is_device_memory(const void* p);
+bool batch_error(hipError_t e, const char* where);
+
+}  // namespace ckfft_gpu

@@ -32,31 +32,16 @@
 #include <vector>
 
 #include "ckfft/ckfft_gpu.h"
+#include "fft_batch.h"
 #include "fft_lds.h"
 #include "fft_pass.h"
 
 namespace ckfft_gpu {
 
-struct BatchItem {
-    int cls;        // class id (lds_cls, or kClsPass + inverse)
-    int first;      // first transform (index into the BatchTransform table)
-    int count;      // transforms in this item (single-workgroup classes)
-    int pad;
-};
-
-struct BatchTransform {
-    const float2* in;
-    float2* out;
-    float2* scratch;   // intermediate of a multi-pass transform (else null)
-};
-
-constexpr int kPersistWG = 512;
 constexpr int kPersistLdsFloats = 34816;        // 139264 bytes
 constexpr int kClsPass = 4096;                  // + inverse
-constexpr int kPassLogN = 16, kPassLogR = 8, kPassLogCB = 5;
+constexpr int kPassLogR = 8, kPassLogCB = 5;
 constexpr int kPassTiles = (1 << (kPassLogN - kPassLogR)) >> kPassLogCB;
-// control words: [0] queue head, [1] workgroups finished
-constexpr int kCtlWords = 2;
 
 __host__ __device__ constexpr int persist_logp(int logc, int var)
 {
@@ -186,29 +171,33 @@ __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchI
 
 // ---- host side ----------------------------------------------------------------------
 
-using namespace ckfft_gpu;
-
-namespace {
+namespace ckfft_gpu {
 
 thread_local char g_batch_error[256] = "no error";
 
-bool berr(hipError_t e, const char* where)
+bool batch_error(hipError_t e, const char* where)
 {
     if (e == hipSuccess) return true;
     std::snprintf(g_batch_error, sizeof(g_batch_error), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
     return false;
 }
 
-bool pow2_positive(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static bool batch_invalid(const char* why, int index)
+{
+    std::snprintf(g_batch_error, sizeof(g_batch_error), "descriptor %d: %s", index, why);
+    return false;
+}
 
-int ilog2(int n)
+static bool pow2_positive(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static int ilog2(int n)
 {
     int l = 0;
     while ((1 << l) < n) ++l;
     return l;
 }
 
-bool is_device_pointer(const void* p)
+bool is_device_memory(const void* p)
 {
     hipPointerAttribute_t a;
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
@@ -218,31 +207,129 @@ bool is_device_pointer(const void* p)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-// bytes one transform reads and writes
 long long transform_bytes(int kind, int log2n)
 {
     const long long n = 1ll << log2n;
     return (kind == kRealForward || kind == kRealInverse) ? 4 * n + 8 * (n / 2 + 1) : 16 * n;
 }
 
-}  // namespace
+bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int index, bool check_pointers)
+{
+    if (d.kind < 0 || d.kind > 3) return batch_invalid("bad kind", index);
+    if (!pow2_positive(d.n) || d.n > ctx->max_count) return batch_invalid("n not a power of two <= nMax", index);
+    const int need_dir = (d.kind == kComplexForward || d.kind == kRealForward) ? kCkFftDirection_Forward
+                                                                               : kCkFftDirection_Inverse;
+    if (!(ctx->direction & need_dir)) return batch_invalid("direction not in the context", index);
+    if (!d.input || !d.output || d.input == d.output) return batch_invalid("null or aliased buffers", index);
+    const int log2n = ilog2(d.n);
+    if (log2n > kLogMaxSupported || log2n > ctx->log_max) return batch_invalid("size not supported", index);
+    if (check_pointers && (!is_device_memory(d.input) || !is_device_memory(d.output)))
+        return batch_invalid("buffers must be device memory", index);
+    return true;
+}
+
+bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
+                 bool check_pointers, BatchBuild& out)
+{
+    out = BatchBuild{};
+    struct Entry {
+        int kind, log2n, logc, idx;
+    };
+    std::vector<Entry> pers;
+    pers.reserve(count);
+    for (int i = 0; i < count; ++i) {
+        const CkFftBatchDesc& d = descs[i];
+        if (!batch_check_desc(ctx, d, i, check_pointers)) return false;
+        const int log2n = ilog2(d.n);
+        const bool real = d.kind >= kRealForward;
+        const int logc = real ? log2n - 1 : log2n;
+        const bool ok = real ? (log2n >= 3 && log2n <= kLogMaxLds + 1) : (log2n >= 1 && log2n <= kPassLogN);
+        if (ok)
+            pers.push_back({d.kind, log2n, logc, i});
+        else
+            out.extras.push_back(BatchExtra{(Kind)d.kind, log2n, d.input, d.output});
+        out.bytes += transform_bytes(d.kind, log2n);
+    }
+    // group by class, keep descriptor order within a class
+    std::stable_sort(pers.begin(), pers.end(), [](const Entry& a, const Entry& b) {
+        return a.kind != b.kind ? a.kind < b.kind : a.logc < b.logc;
+    });
+    out.trs.resize(pers.size());
+    std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item)
+    for (size_t i = 0; i < pers.size(); ++i) {
+        const CkFftBatchDesc& d = descs[pers[i].idx];
+        out.trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr};
+    }
+    for (size_t i = 0; i < pers.size();) {
+        const Entry& e = pers[i];
+        if (e.logc == kPassLogN) {
+            out.twopass.push_back((int)i);
+            const int inv = e.kind == kComplexInverse ? 1 : 0;
+            // both passes move the signal: 4 x 16 bytes per point
+            work.push_back({64ll << kPassLogN, BatchItem{kClsPass + inv, (int)i, 1, 0}});
+            ++i;
+            continue;
+        }
+        const int var = var_override >= 0 ? var_override : ctx->tune[e.kind][e.logc];
+        const int T = 1 << (e.logc - persist_logp(e.logc, var));
+        const int tpw = kPersistWG / T;
+        size_t j = i;
+        while (j < pers.size() && pers[j].kind == e.kind && pers[j].logc == e.logc && (int)(j - i) < tpw) ++j;
+        const int cnt = (int)(j - i);
+        work.push_back({cnt * transform_bytes(e.kind, e.log2n), BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
+        i = j;
+    }
+    // largest first, so the tail of the queue is short work
+    std::stable_sort(work.begin(), work.end(),
+                     [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
+                         return a.first > b.first;
+                     });
+    out.items.reserve(work.size());
+    for (auto& w : work) out.items.push_back(w.second);
+    out.persistent = (int)pers.size();
+    return true;
+}
+
+void batch_assign_scratch(BatchBuild& b, float2* scratch)
+{
+    for (size_t k = 0; k < b.twopass.size(); ++k) b.trs[b.twopass[k]].scratch = scratch + (k << kPassLogN);
+}
+
+int persistent_grid(const CkFftContext* ctx)
+{
+    int per_cu = 0, cus = 0;
+    if (!batch_error(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_persistent_kernel, kPersistWG, 0),
+                     "occupancy") ||
+        !batch_error(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count"))
+        return 0;
+    return std::max(1, per_cu * cus);
+}
+
+hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
+                                 int grid, const DeviceTables& tb, hipStream_t s)
+{
+    hipLaunchKernelGGL(fft_persistent_kernel, dim3(grid), dim3(kPersistWG), 0, s, d_items, n_items, d_trs, d_ctl, tb);
+    return hipGetLastError();
+}
+
+DeviceTables context_tables(const CkFftContext* c)
+{
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+}
+
+}  // namespace ckfft_gpu
+
+using namespace ckfft_gpu;
 
 struct _CkFftBatchPlan {
     CkFftContext* ctx;
     int n_items = 0;
-    int n_trs = 0;
     int grid = 0;
     BatchItem* d_items = nullptr;
     BatchTransform* d_trs = nullptr;
     int* d_ctl = nullptr;
     float2* d_scratch = nullptr;
-    struct Extra {
-        Kind kind;
-        int log2n;
-        const void* in;
-        void* out;
-    };
-    std::vector<Extra> extras;
+    std::vector<BatchExtra> extras;
     bool use_graph = false;
     bool graph_ready = false;
     hipGraphExec_t gexec = nullptr;
@@ -256,107 +343,38 @@ namespace {
 CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int flags, int var_override)
 {
     if (!ctx || ctx->magic != kContextMagic || count < 0 || (count > 0 && !descs)) return nullptr;
+    BatchBuild b;
+    if (!batch_build(ctx, descs, count, var_override, true, b)) return nullptr;
     auto* plan = new (std::nothrow) CkFftBatchPlan();
     if (!plan) return nullptr;
     plan->ctx = ctx;
     plan->use_graph = (flags & kCkFftBatch_Graph) != 0;
-
-    struct Entry {
-        int kind, log2n, logc, idx;
-    };
-    std::vector<Entry> pers;
-    for (int i = 0; i < count; ++i) {
-        const CkFftBatchDesc& d = descs[i];
-        if (d.kind < 0 || d.kind > 3 || !pow2_positive(d.n) || d.n > ctx->max_count) goto bad;
-        {
-            const int need_dir = (d.kind == kComplexForward || d.kind == kRealForward) ? kCkFftDirection_Forward
-                                                                                       : kCkFftDirection_Inverse;
-            if (!(ctx->direction & need_dir)) goto bad;
-        }
-        if (!d.input || !d.output || d.input == d.output) goto bad;
-        if (!is_device_pointer(d.input) || !is_device_pointer(d.output)) goto bad;
-        {
-            const int log2n = ilog2(d.n);
-            if (log2n > kLogMaxSupported || log2n > ctx->log_max) goto bad;
-            const bool real = d.kind >= kRealForward;
-            const int logc = real ? log2n - 1 : log2n;
-            const bool ok = real ? (log2n >= 3 && log2n <= kLogMaxLds + 1) : (log2n >= 1 && log2n <= kPassLogN);
-            if (ok)
-                pers.push_back({d.kind, log2n, logc, i});
-            else
-                plan->extras.push_back({(Kind)d.kind, log2n, d.input, d.output});
-        }
-    }
-    {
-        // group by class, keep descriptor order within a class
-        std::stable_sort(pers.begin(), pers.end(), [](const Entry& a, const Entry& b) {
-            return a.kind != b.kind ? a.kind < b.kind : a.logc < b.logc;
-        });
-        std::vector<BatchTransform> trs(pers.size());
-        std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item)
-        size_t n_two = 0;
-        for (size_t i = 0; i < pers.size(); ++i) {
-            const CkFftBatchDesc& d = descs[pers[i].idx];
-            trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr};
-            if (pers[i].logc == kPassLogN) ++n_two;
-        }
-        if (n_two && !berr(hipMalloc(&plan->d_scratch, sizeof(float2) * (n_two << kPassLogN)), "hipMalloc(batch scratch)"))
+    plan->extras = b.extras;
+    plan->n_items = (int)b.items.size();
+    if (!b.twopass.empty() &&
+        !batch_error(hipMalloc(&plan->d_scratch, sizeof(float2) * (b.twopass.size() << kPassLogN)), "hipMalloc(scratch)"))
+        goto bad;
+    batch_assign_scratch(b, plan->d_scratch);
+    if (plan->n_items) {
+        if (!batch_error(hipMalloc(&plan->d_items, sizeof(BatchItem) * b.items.size()), "hipMalloc(items)") ||
+            !batch_error(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * b.trs.size()), "hipMalloc(transforms)") ||
+            !batch_error(hipMalloc(&plan->d_ctl, sizeof(int) * kCtlWords), "hipMalloc(control)") ||
+            !batch_error(hipMemcpy(plan->d_items, b.items.data(), sizeof(BatchItem) * b.items.size(),
+                                   hipMemcpyHostToDevice), "upload items") ||
+            !batch_error(hipMemcpy(plan->d_trs, b.trs.data(), sizeof(BatchTransform) * b.trs.size(),
+                                   hipMemcpyHostToDevice), "upload transforms") ||
+            !batch_error(hipMemset(plan->d_ctl, 0, sizeof(int) * kCtlWords), "clear control"))
             goto bad;
-        n_two = 0;
-        for (size_t i = 0; i < pers.size();) {
-            const Entry& e = pers[i];
-            if (e.logc == kPassLogN) {
-                trs[i].scratch = plan->d_scratch + (n_two++ << kPassLogN);
-                const int inv = e.kind == kComplexInverse ? 1 : 0;
-                // both passes move the signal: 4 x 16 bytes per point
-                work.push_back({64ll << kPassLogN, BatchItem{kClsPass + inv, (int)i, 1, 0}});
-                ++i;
-                continue;
-            }
-            const int var = var_override >= 0 ? var_override : ctx->tune[e.kind][e.logc];
-            const int T = 1 << (e.logc - persist_logp(e.logc, var));
-            const int tpw = kPersistWG / T;
-            size_t j = i;
-            while (j < pers.size() && pers[j].kind == e.kind && pers[j].logc == e.logc && (int)(j - i) < tpw) ++j;
-            const int cnt = (int)(j - i);
-            work.push_back({cnt * transform_bytes(e.kind, e.log2n),
-                            BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
-            i = j;
-        }
-        std::stable_sort(work.begin(), work.end(),
-                         [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
-                             return a.first > b.first;
-                         });
-        std::vector<BatchItem> items;
-        for (auto& w : work) items.push_back(w.second);
-        plan->n_items = (int)items.size();
-        plan->n_trs = (int)trs.size();
-        if (plan->n_items) {
-            if (!berr(hipMalloc(&plan->d_items, sizeof(BatchItem) * items.size()), "hipMalloc(items)") ||
-                !berr(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * trs.size()), "hipMalloc(transforms)") ||
-                !berr(hipMalloc(&plan->d_ctl, sizeof(int) * kCtlWords), "hipMalloc(control)") ||
-                !berr(hipMemcpy(plan->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice),
-                      "upload items") ||
-                !berr(hipMemcpy(plan->d_trs, trs.data(), sizeof(BatchTransform) * trs.size(), hipMemcpyHostToDevice),
-                      "upload transforms") ||
-                !berr(hipMemset(plan->d_ctl, 0, sizeof(int) * kCtlWords), "clear control"))
-                goto bad;
-            int per_cu = 0, cus = 0;
-            if (!berr(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_persistent_kernel, kPersistWG, 0),
-                      "occupancy") ||
-                !berr(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count"))
-                goto bad;
-            plan->grid = std::max(1, std::min(per_cu * cus, plan->n_items));
-        }
-        plan->info.transforms = count;
-        plan->info.persistent_transforms = (int)pers.size();
-        plan->info.work_items = plan->n_items;
-        plan->info.workgroups = plan->grid;
-        plan->info.extra_launches = (int)plan->extras.size();
-        long long bytes = 0;
-        for (int i = 0; i < count; ++i) bytes += transform_bytes(descs[i].kind, ilog2(descs[i].n));
-        plan->info.algorithmic_bytes = bytes;
+        const int g = persistent_grid(ctx);
+        if (!g) goto bad;
+        plan->grid = std::min(g, plan->n_items);
     }
+    plan->info.transforms = count;
+    plan->info.persistent_transforms = b.persistent;
+    plan->info.work_items = plan->n_items;
+    plan->info.workgroups = plan->grid;
+    plan->info.extra_launches = (int)plan->extras.size();
+    plan->info.algorithmic_bytes = b.bytes;
     return plan;
 bad:
     CkFftBatchPlanDestroy(plan);
@@ -365,11 +383,7 @@ bad:
 
 hipError_t launch_persistent(CkFftBatchPlan* p, hipStream_t s)
 {
-    const CkFftContext* c = p->ctx;
-    DeviceTables tb{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
-    hipLaunchKernelGGL(fft_persistent_kernel, dim3(p->grid), dim3(kPersistWG), 0, s, p->d_items, p->n_items, p->d_trs,
-                       p->d_ctl, tb);
-    return hipGetLastError();
+    return launch_persistent_raw(p->d_items, p->n_items, p->d_trs, p->d_ctl, p->grid, context_tables(p->ctx), s);
 }
 
 }  // namespace
@@ -389,24 +403,23 @@ int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream)
         if (plan->use_graph) {
             if (!plan->graph_ready) {
                 hipStream_t cap = nullptr;
-                if (!berr(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream")) return 0;
-                bool ok = berr(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture") &&
-                          berr(launch_persistent(plan, cap), "capture launch") &&
-                          berr(hipStreamEndCapture(cap, &plan->graph), "end capture") &&
-                          berr(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0), "instantiate");
+                if (!batch_error(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream")) return 0;
+                bool ok = batch_error(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture") &&
+                          batch_error(launch_persistent(plan, cap), "capture launch") &&
+                          batch_error(hipStreamEndCapture(cap, &plan->graph), "end capture") &&
+                          batch_error(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0), "instantiate");
                 (void)hipStreamDestroy(cap);
                 if (!ok) return 0;
                 plan->graph_ready = true;
             }
-            if (!berr(hipGraphLaunch(plan->gexec, s), "graph launch")) return 0;
-        } else if (!berr(launch_persistent(plan, s), "persistent launch")) {
+            if (!batch_error(hipGraphLaunch(plan->gexec, s), "graph launch")) return 0;
+        } else if (!batch_error(launch_persistent(plan, s), "persistent launch")) {
             return 0;
         }
     }
-    const CkFftContext* c = plan->ctx;
-    DeviceTables tb{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+    const DeviceTables tb = context_tables(plan->ctx);
     for (const auto& e : plan->extras)
-        if (!berr(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, s), "extra launch")) return 0;
+        if (!batch_error(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, s), "extra launch")) return 0;
     return 1;
 }
 
@@ -441,9 +454,9 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath)
     const size_t kBytesPerClass = 32u << 20;
     void* in = nullptr;
     void* out = nullptr;
-    if (!berr(hipMalloc(&in, kBytesPerClass), "hipMalloc(tune in)") ||
-        !berr(hipMalloc(&out, kBytesPerClass + 4096 * 8), "hipMalloc(tune out)") ||
-        !berr(hipMemset(in, 0, kBytesPerClass), "clear tune input")) {
+    if (!batch_error(hipMalloc(&in, kBytesPerClass), "hipMalloc(tune in)") ||
+        !batch_error(hipMalloc(&out, kBytesPerClass + 4096 * 8), "hipMalloc(tune out)") ||
+        !batch_error(hipMemset(in, 0, kBytesPerClass), "clear tune input")) {
         if (in) (void)hipFree(in);
         return 0;
     }

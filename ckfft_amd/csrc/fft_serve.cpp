@@ -1,0 +1,444 @@
+// fft_serve.cpp -- continuous-batching serving loop over the persistent batch kernel.
+//
+// The reference has no serving path: a caller runs one transform per synchronous call
+// (src/ckfft/ckfft.cpp:36-114).  Here client threads submit transforms (device buffers)
+// at any time; one dispatcher thread
+//  * admits queued requests first-come-first-served while the bytes of admitted,
+//    unfinished requests stay within the HBM budget (back-pressure, not failure);
+//  * whenever one of `max_in_flight` slots is free, launches every admitted request
+//    as one batch -- one persistent-kernel launch on the slot's own stream, descriptors
+//    uploaded asynchronously from the slot's pinned buffers -- so batches formed while
+//    others run overlap on the device;
+//  * polls the slots' completion events, finishes their requests and frees budget.
+// Twiddles, descriptor buffers, scratch and streams stay resident for the server's
+// life.  A request's result does not depend on the batch it ran in: every kernel that
+// computes a transform rounds the same way.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "ckfft/ckfft_gpu.h"
+#include "fft_batch.h"
+
+using namespace ckfft_gpu;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
+}
+
+enum ReqState : int { kQueued = 0, kAdmitted = 1, kLaunched = 2, kDone = 3, kFailed = 4 };
+
+struct Request {
+    CkFftBatchDesc d;
+    long long bytes;        // algorithmic bytes, charged against the budget
+    uint64_t t_arrive;      // arrival (submission or scheduled arrival) time
+    uint64_t t_done;
+    std::atomic<int> state;
+    Request(const CkFftBatchDesc& dd, long long b, uint64_t t) : d(dd), bytes(b), t_arrive(t), t_done(0), state(kQueued)
+    {
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    char* h_desc = nullptr;              // pinned staging of the batch's queue and transform table
+    char* d_desc = nullptr;              // their device copy
+    int* d_ctl = nullptr;
+    float2* d_scratch = nullptr;
+    std::vector<long long> reqs;
+    long long bytes = 0;
+    bool busy = false;
+};
+
+constexpr int kMaxBatch = 2048;          // transforms per batch (slot buffer capacity)
+constexpr int kScratchPerSlot = 32;      // complex 2^16 transforms per batch
+constexpr uint64_t kSpinNs = 2000000;    // idle dispatcher spins this long before sleeping
+// queue entries first, the transform table after them at a 256-byte boundary, so one
+// copy uploads both
+constexpr size_t kDescBytes = sizeof(BatchItem) * kMaxBatch + 256 + sizeof(BatchTransform) * kMaxBatch;
+
+}  // namespace
+
+struct _CkFftServer {
+    CkFftContext* ctx = nullptr;
+    long long budget = 0;                // bytes available to requests
+    long long resident = 0;              // bytes the server itself holds on the device
+    int grid = 0;
+    std::vector<Slot> slots;
+    std::deque<Request> reqs;            // ticket = index; deque keeps addresses stable
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<long long> queued;        // submitted, not admitted
+    std::vector<long long> admitted;     // admitted, not launched
+    long long inflight_bytes = 0;
+    long long peak_inflight = 0;
+    long long outstanding = 0;
+    std::atomic<bool> stop{false};
+    std::thread worker;
+    // statistics (guarded by m)
+    std::vector<double> lat_us;
+    long long completed = 0, failed = 0, batches = 0, batched = 0, done_bytes = 0;
+    uint64_t t_first = 0, t_last = 0;
+};
+
+namespace {
+
+thread_local char g_serve_error[256] = "no error";
+
+bool serr(hipError_t e, const char* where)
+{
+    if (e == hipSuccess) return true;
+    std::snprintf(g_serve_error, sizeof(g_serve_error), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+    return false;
+}
+
+void finish_slot(CkFftServer* S, Slot& sl, bool ok)
+{
+    const uint64_t t = now_ns();
+    {
+        std::lock_guard<std::mutex> lock(S->m);
+        for (long long id : sl.reqs) {
+            Request& r = S->reqs[id];
+            r.t_done = t;
+            if (ok) {
+                S->lat_us.push_back((double)(t - r.t_arrive) * 1e-3);
+                ++S->completed;
+                S->done_bytes += r.bytes;
+            } else {
+                ++S->failed;
+            }
+            r.state.store(ok ? kDone : kFailed);
+        }
+        S->inflight_bytes -= sl.bytes;
+        S->outstanding -= (long long)sl.reqs.size();
+        S->t_last = t;
+    }
+    sl.reqs.clear();
+    sl.bytes = 0;
+    sl.busy = false;
+    S->cv.notify_all();
+}
+
+// `descs` and `bytes` were copied from the requests `batch` under the lock (the request
+// deque may grow concurrently, so it is only indexed under the lock).
+bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const std::vector<CkFftBatchDesc>& descs,
+                  long long bytes)
+{
+    BatchBuild b;
+    // descriptors were validated at submission
+    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b)) return false;
+    batch_assign_scratch(b, sl.d_scratch);
+    const DeviceTables tb = context_tables(S->ctx);
+    bool ok = true;
+    if (!b.items.empty()) {
+        const size_t items_bytes = sizeof(BatchItem) * b.items.size();
+        const size_t trs_off = (items_bytes + 255) & ~(size_t)255;
+        const size_t total = trs_off + sizeof(BatchTransform) * b.trs.size();
+        std::memcpy(sl.h_desc, b.items.data(), items_bytes);
+        std::memcpy(sl.h_desc + trs_off, b.trs.data(), sizeof(BatchTransform) * b.trs.size());
+        ok = serr(hipMemcpyAsync(sl.d_desc, sl.h_desc, total, hipMemcpyHostToDevice, sl.stream), "upload queue") &&
+             serr(launch_persistent_raw((const BatchItem*)sl.d_desc, (int)b.items.size(),
+                                        (const BatchTransform*)(sl.d_desc + trs_off), sl.d_ctl,
+                                        std::min(S->grid, (int)b.items.size()), tb, sl.stream),
+                  "persistent launch");
+    }
+    for (const auto& e : b.extras)
+        if (ok) ok = serr(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, sl.stream), "extra launch");
+    if (ok) ok = serr(hipEventRecord(sl.done, sl.stream), "record");
+    sl.reqs.swap(batch);
+    sl.bytes = bytes;
+    sl.busy = true;
+    {
+        std::lock_guard<std::mutex> lock(S->m);
+        for (long long id : sl.reqs) S->reqs[id].state.store(kLaunched);
+        ++S->batches;
+        S->batched += (long long)sl.reqs.size();
+    }
+    if (!ok) finish_slot(S, sl, false);
+    return ok;
+}
+
+void dispatcher(CkFftServer* S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::vector<long long> batch;
+    std::vector<CkFftBatchDesc> descs;
+    long long bytes = 0;
+    batch.reserve(kMaxBatch);
+    descs.reserve(kMaxBatch);
+    uint64_t idle_since = 0;
+    for (;;) {
+        bool progress = false;
+        for (Slot& sl : S->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipSuccess) {
+                finish_slot(S, sl, true);
+                progress = true;
+            } else if (q != hipErrorNotReady) {
+                serr(q, "batch completion");
+                finish_slot(S, sl, false);
+                progress = true;
+            }
+        }
+        Slot* free_slot = nullptr;
+        for (Slot& sl : S->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        {
+            std::unique_lock<std::mutex> lock(S->m);
+            while (!S->queued.empty()) {
+                Request& r = S->reqs[S->queued.front()];
+                if (S->inflight_bytes + r.bytes > S->budget) break;      // back-pressure
+                S->inflight_bytes += r.bytes;
+                S->peak_inflight = std::max(S->peak_inflight, S->inflight_bytes);
+                r.state.store(kAdmitted);
+                S->admitted.push_back(S->queued.front());
+                S->queued.pop_front();
+            }
+            if (free_slot && !S->admitted.empty()) {
+                // take admitted requests in order, within the slot's capacities
+                size_t take = 0;
+                int two = 0;
+                while (take < S->admitted.size() && (int)take < kMaxBatch) {
+                    const CkFftBatchDesc& d = S->reqs[S->admitted[take]].d;
+                    if (d.kind <= kComplexInverse && d.n == (1 << kPassLogN)) {
+                        if (two == kScratchPerSlot) break;
+                        ++two;
+                    }
+                    ++take;
+                }
+                batch.assign(S->admitted.begin(), S->admitted.begin() + take);
+                S->admitted.erase(S->admitted.begin(), S->admitted.begin() + take);
+                descs.clear();
+                bytes = 0;
+                for (long long id : batch) {
+                    descs.push_back(S->reqs[id].d);
+                    bytes += S->reqs[id].bytes;
+                }
+            }
+            if (batch.empty() && !progress && S->outstanding == 0) {
+                if (S->stop.load()) return;
+                // nothing submitted: spin briefly (a request arriving now is batched at
+                // once), then sleep until a submission or shutdown
+                const uint64_t t = now_ns();
+                if (!idle_since) idle_since = t;
+                if (t - idle_since > kSpinNs) {
+                    S->cv.wait(lock, [&] { return !S->queued.empty() || S->stop.load(); });
+                    idle_since = 0;
+                    continue;
+                }
+            } else {
+                idle_since = 0;
+            }
+        }
+        if (!batch.empty()) {
+            launch_batch(S, *free_slot, batch, descs, bytes);
+            batch.clear();
+            progress = true;
+        }
+        if (!progress) std::this_thread::yield();
+    }
+}
+
+void destroy_slots(CkFftServer* S)
+{
+    for (Slot& sl : S->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.h_desc) (void)hipHostFree(sl.h_desc);
+        if (sl.d_desc) (void)hipFree(sl.d_desc);
+        if (sl.d_ctl) (void)hipFree(sl.d_ctl);
+        if (sl.d_scratch) (void)hipFree(sl.d_scratch);
+    }
+    S->slots.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxInFlight)
+{
+    if (!context || context->magic != kContextMagic || maxInFlight < 1 || maxInFlight > 64) return nullptr;
+    auto* S = new (std::nothrow) CkFftServer();
+    if (!S) return nullptr;
+    S->ctx = context;
+    S->slots.resize(maxInFlight);
+    const bool two_pass = context->log_max >= kPassLogN;
+    const size_t scratch_bytes = two_pass ? sizeof(float2) * ((size_t)kScratchPerSlot << kPassLogN) : 0;
+    for (Slot& sl : S->slots) {
+        if (!serr(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "stream") ||
+            !serr(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "event") ||
+            !serr(hipHostMalloc((void**)&sl.h_desc, kDescBytes, hipHostMallocDefault), "pinned queue") ||
+            !serr(hipMalloc((void**)&sl.d_desc, kDescBytes), "queue") ||
+            !serr(hipMalloc(&sl.d_ctl, sizeof(int) * kCtlWords), "control") ||
+            !serr(hipMemset(sl.d_ctl, 0, sizeof(int) * kCtlWords), "clear control") ||
+            (scratch_bytes && !serr(hipMalloc(&sl.d_scratch, scratch_bytes), "scratch"))) {
+            destroy_slots(S);
+            delete S;
+            return nullptr;
+        }
+        S->resident += (long long)(scratch_bytes + kDescBytes + sizeof(int) * kCtlWords);
+    }
+    S->budget = (long long)hbmBudget - S->resident;
+    S->grid = persistent_grid(context);
+    if (S->budget <= 0 || S->grid <= 0) {
+        std::snprintf(g_serve_error, sizeof(g_serve_error), "budget below the server's resident %lld bytes",
+                      S->resident);
+        destroy_slots(S);
+        delete S;
+        return nullptr;
+    }
+    S->worker = std::thread(dispatcher, S);
+    return S;
+}
+
+static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_arrive)
+{
+    if (!batch_check_desc(S->ctx, d, 0, true)) {
+        std::snprintf(g_serve_error, sizeof(g_serve_error), "invalid request (%s)", CkFftBatchLastError());
+        return -1;
+    }
+    int log2n = 0;
+    while ((1 << log2n) < d.n) ++log2n;
+    const long long bytes = transform_bytes(d.kind, log2n);
+    if (bytes > S->budget) {
+        std::snprintf(g_serve_error, sizeof(g_serve_error), "request larger than the admission budget");
+        return -1;
+    }
+    long long id;
+    {
+        std::lock_guard<std::mutex> lock(S->m);
+        id = (long long)S->reqs.size();
+        S->reqs.emplace_back(d, bytes, t_arrive);
+        S->queued.push_back(id);
+        ++S->outstanding;
+        if (!S->t_first) S->t_first = t_arrive;
+    }
+    S->cv.notify_all();
+    return id;
+}
+
+long long CkFftServerSubmit(CkFftServer* server, int kind, int n, const void* input, void* output)
+{
+    if (!server) return -1;
+    return submit_at(server, CkFftBatchDesc{kind, n, input, output}, now_ns());
+}
+
+int CkFftServerReplay(CkFftServer* server, const CkFftBatchDesc* descs, const double* arrivalUs, int count,
+                      long long* tickets)
+{
+    if (!server || count < 0 || (count && (!descs || !arrivalUs))) return 0;
+    const uint64_t t0 = now_ns();
+    int ok = 1;
+    for (int i = 0; i < count; ++i) {
+        const uint64_t t = t0 + (uint64_t)(arrivalUs[i] * 1e3);
+        while (now_ns() < t) {
+        }
+        const long long id = submit_at(server, descs[i], t);
+        if (tickets) tickets[i] = id;
+        if (id < 0) ok = 0;
+    }
+    return ok;
+}
+
+int CkFftServerQuery(CkFftServer* server, long long ticket)
+{
+    if (!server || ticket < 0) return -1;
+    std::lock_guard<std::mutex> lock(server->m);
+    if (ticket >= (long long)server->reqs.size()) return -1;
+    const int st = server->reqs[ticket].state.load();
+    return st == kDone ? 1 : (st == kFailed ? -1 : 0);
+}
+
+int CkFftServerWait(CkFftServer* server, long long ticket)
+{
+    if (!server || ticket < 0) return 0;
+    std::unique_lock<std::mutex> lock(server->m);
+    if (ticket >= (long long)server->reqs.size()) return 0;
+    Request& r = server->reqs[ticket];
+    server->cv.wait(lock, [&] { return r.state.load() >= kDone; });
+    return r.state.load() == kDone ? 1 : 0;
+}
+
+int CkFftServerDrain(CkFftServer* server)
+{
+    if (!server) return 0;
+    std::unique_lock<std::mutex> lock(server->m);
+    server->cv.wait(lock, [&] { return server->outstanding == 0; });
+    return server->failed == 0 ? 1 : 0;
+}
+
+int CkFftServerGetStats(CkFftServer* server, CkFftServerStats* out)
+{
+    if (!server || !out) return 0;
+    std::lock_guard<std::mutex> lock(server->m);
+    std::vector<double> l = server->lat_us;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+    double mean = 0;
+    for (double v : l) mean += v;
+    *out = CkFftServerStats{};
+    out->completed = server->completed;
+    out->failed = server->failed;
+    out->p50_us = pct(0.50);
+    out->p99_us = pct(0.99);
+    out->max_us = l.empty() ? 0.0 : l.back();
+    out->mean_us = l.empty() ? 0.0 : mean / l.size();
+    const double span = server->t_last > server->t_first ? (server->t_last - server->t_first) * 1e-9 : 0.0;
+    out->requests_per_s = span > 0 ? server->completed / span : 0.0;
+    out->GB_per_s = span > 0 ? server->done_bytes / span * 1e-9 : 0.0;
+    out->peak_inflight_bytes = server->peak_inflight;
+    out->budget_bytes = server->budget;
+    out->resident_bytes = server->resident;
+    out->batches = server->batches;
+    out->mean_batch = server->batches ? (double)server->batched / server->batches : 0.0;
+    return 1;
+}
+
+void CkFftServerResetStats(CkFftServer* server)
+{
+    if (!server) return;
+    std::lock_guard<std::mutex> lock(server->m);
+    server->lat_us.clear();
+    server->completed = server->failed = server->batches = server->batched = server->done_bytes = 0;
+    server->peak_inflight = server->inflight_bytes;
+    server->t_first = server->t_last = 0;
+}
+
+void CkFftServerDestroy(CkFftServer* server)
+{
+    if (!server) return;
+    {
+        std::lock_guard<std::mutex> lock(server->m);
+        server->stop.store(true);
+    }
+    server->cv.notify_all();
+    if (server->worker.joinable()) server->worker.join();   // finishes outstanding requests first
+    destroy_slots(server);
+    delete server;
+}
+
+const char* CkFftServerLastError(void) { return g_serve_error; }
+
+}  // extern "C"

@@ -91,6 +91,59 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath);
 /* Load choices written by CkFftAutotune.  1 if any entry was applied. */
 int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
 
+/* ---- serving loop ------------------------------------------------------------------
+ *
+ * A server admits transforms as they arrive, from any thread, and runs them with
+ * continuous batching: a dispatcher thread takes every admitted request into the next
+ * free slot (up to maxInFlight batches on the device at once, each one persistent
+ * launch on the slot's own stream), so requests arriving while others run join the
+ * next batch and their work overlaps on the device.  Admission is first-come-first-
+ * served against an HBM budget: a request is admitted while the algorithmic bytes of
+ * admitted, unfinished requests (one read of the input, one write of the output) plus
+ * the server's own resident buffers stay within hbmBudget; others wait (back-pressure,
+ * not failure).  Twiddles, descriptor buffers, scratch and streams stay resident.
+ * A request's result is bit-identical to the single-signal call on the same input,
+ * whatever it was batched with.  Buffers are device memory owned by the caller and
+ * must stay valid until the request completes.  The reference has no counterpart. */
+
+typedef struct _CkFftServer CkFftServer;
+
+typedef struct {
+    long long completed;          /* requests finished since the last reset */
+    long long failed;
+    double p50_us, p99_us, max_us, mean_us;  /* latency, arrival to completion seen */
+    double requests_per_s;        /* completed / (last completion - first arrival) */
+    double GB_per_s;              /* algorithmic bytes over the same span */
+    long long peak_inflight_bytes;/* largest admitted-unfinished byte count */
+    long long budget_bytes;       /* hbmBudget minus the resident bytes */
+    long long resident_bytes;     /* the server's own device buffers */
+    long long batches;            /* batches launched */
+    double mean_batch;            /* requests per batch */
+} CkFftServerStats;
+
+/* NULL on invalid arguments, a budget below the server's resident buffers, or
+ * allocation failure (CkFftServerLastError tells which).  maxInFlight in [1, 64]. */
+CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxInFlight);
+/* Submit one transform (kind, n and buffers as in CkFftBatchDesc).  Returns a ticket
+ * >= 0, or -1 if the request is invalid or alone exceeds the budget. */
+long long CkFftServerSubmit(CkFftServer* server, int kind, int n, const void* input, void* output);
+/* Submit descs[i] at arrivalUs[i] microseconds after the call (non-decreasing times),
+ * from the calling thread, writing tickets[i] (may be NULL).  Latency is measured from
+ * the scheduled arrival.  1 if every request was accepted. */
+int CkFftServerReplay(CkFftServer* server, const CkFftBatchDesc* descs, const double* arrivalUs, int count,
+                      long long* tickets);
+/* 1 done, 0 pending, -1 failed or unknown ticket. */
+int CkFftServerQuery(CkFftServer* server, long long ticket);
+/* Block until the ticket completes; 1 on success. */
+int CkFftServerWait(CkFftServer* server, long long ticket);
+/* Block until every submitted request completes; 1 if none failed. */
+int CkFftServerDrain(CkFftServer* server);
+int CkFftServerGetStats(CkFftServer* server, CkFftServerStats* stats);
+void CkFftServerResetStats(CkFftServer* server);
+/* Finishes outstanding requests, then frees the server. */
+void CkFftServerDestroy(CkFftServer* server);
+const char* CkFftServerLastError(void);
+
 /* Copy the device twiddle table of level L (L values exp(-2 pi i k / L), k < L,
  * generated in FP64 on the GPU and rounded to FP32) to host memory `out`.
  * L must be a power of two no larger than the context's table.  Returns 1/0. */
