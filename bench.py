@@ -17,7 +17,9 @@ Also reported on the same line:
   * parity: a sample of the benchmarked outputs against the CPU oracle and FP64;
   * serving: Poisson arrivals at 200k requests/s of the uneven mix through the
     continuous-batching server under a 4 GiB HBM budget: p50/p99 latency, throughput,
-    and a bitwise alone-vs-mix check on a sample of the outputs.
+    and a bitwise alone-vs-mix check on a sample of the outputs;
+  * split_bf16x3: the opt-in matrix-core precision mode at N = 4096..65536 (256 MiB
+    batches): throughput, matrix-core FLOP/s and error against FP64, FP32 mode beside it.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -272,6 +274,58 @@ def serving_leg(torch, C, dev, rate: float = 200e3, seconds: float = 2.0, budget
     return out
 
 
+def split_bf16_leg(torch, C, O, dev, reps: int = 20, sizes=(4096, 8192, 16384, 32768, 65536),
+                   in_bytes: int = 256 << 20):
+    """Opt-in split-BF16x3 mode: complex forward transforms of N = 4096..65536, batched to
+    256 MiB of input, radix-16 stages on the matrix cores.  Per N: launch time (median of
+    `reps` event-timed launches), GFLOP/s (5 N log2 N), algorithmic GB/s (16 N per
+    transform) against 8 TB/s, matrix-core TFLOP/s actually issued (12 MFMA of 32x32x16
+    per 32 radix-16 butterflies: 768 N flops per radix-16 stage, log2(N)/4 stages), the
+    relative L2 error against the FP64 DFT on a sample of signals, and the same for the
+    FP32 mode beside it."""
+    stream = torch.cuda.current_stream(dev)
+    rows = []
+    for n in sizes:
+        count = in_bytes // (8 * n)
+        logn = int(math.log2(n))
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(0x5B + logn)
+        x = torch.rand(count, 2 * n, device=dev, generator=gen) * 2 - 1
+        y = torch.empty_like(x)
+        row = {"n": n, "count": count}
+        for mode, name in ((C.kCkFftPrecision_SplitBf16x3, "split_bf16x3"), (C.kCkFftPrecision_Fp32, "fp32")):
+            ctx = C.Context(n, C.kCkFftDirection_Both)
+            assert ctx.set_precision(mode) == 1
+            for _ in range(3):
+                assert ctx.complex_forward_batch(n, count, x, y, stream) == 1
+            torch.cuda.synchronize()
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+            for a, b in ev:
+                a.record(stream)
+                ctx.complex_forward_batch(n, count, x, y, stream)
+                b.record(stream)
+            torch.cuda.synchronize()
+            ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
+            idx = [0, count // 2, count - 1]
+            xs = x[idx].cpu().numpy()
+            err = O.rel_l2(y[idx].cpu().numpy(), O.interleave(O.fp64_complex(xs, False)))
+            gbs = 16.0 * n * count / (ms * 1e-3) / 1e9
+            r = {"ms": round(ms, 4), "GFLOP_s": round(5.0 * n * logn * count / (ms * 1e-3) / 1e9, 1),
+                 "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                 "relL2_fp64": float(f"{err:.3e}")}
+            if mode == C.kCkFftPrecision_SplitBf16x3:
+                mfma_flops = 768.0 * n * (logn // 4) * count
+                r["mfma_TFLOP_s"] = round(mfma_flops / (ms * 1e-3) / 1e12, 1)
+                r["mfma_frac_of_bf16_peak"] = round(mfma_flops / (ms * 1e-3) / 2.5e15, 4)
+            row[name] = r
+            ctx.close()
+        rows.append(row)
+        del x, y
+    return {"workload": "complex forward, N=4096..65536, 256 MiB of input per N, iid U[-1,1]",
+            "mfma_flops_convention": "768 N per radix-16 stage (6 split products x 2 K-halves x 32x32x16)",
+            "gate": "relL2_fp64 <= 1e-6", "sizes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +335,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-signal leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (persistent launch) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-BF16x3 precision leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
                          "HBM counters FETCH_SIZE / WRITE_SIZE")
@@ -400,6 +455,7 @@ def main():
     del x, y, z
     uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=2)
     serving = None if args.no_serve else serving_leg(torch, C, dev)
+    split = None if args.no_split else split_bf16_leg(torch, C, O, dev)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
@@ -415,6 +471,7 @@ def main():
         "parity": parity,
         "batched_uneven": uneven,
         "serving": serving,
+        "split_bf16x3": split,
         "device": torch.cuda.get_device_name(dev),
     }
     print(json.dumps(line))

@@ -24,7 +24,8 @@ __all__ = [
     "CkFftBatchPlanDestroy", "CkFftBatchLastError", "CkFftAutotune", "CkFftLoadTuning",
     "CkFftServerStats", "CkFftServerCreate", "CkFftServerSubmit", "CkFftServerReplay", "CkFftServerQuery",
     "CkFftServerWait", "CkFftServerDrain", "CkFftServerGetStats", "CkFftServerResetStats",
-    "CkFftServerDestroy", "CkFftServerLastError", "Server", "EXPORTED_SYMBOLS",
+    "CkFftServerDestroy", "CkFftServerLastError", "Server", "CkFftSetPrecision", "CkFftGetPrecision",
+    "kCkFftPrecision_Fp32", "kCkFftPrecision_SplitBf16x3", "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
@@ -59,6 +60,8 @@ kCkFftKind_ComplexInverse = 1
 kCkFftKind_RealForward = 2
 kCkFftKind_RealInverse = 3
 kCkFftBatch_Graph = 1
+kCkFftPrecision_Fp32 = 0
+kCkFftPrecision_SplitBf16x3 = 1
 
 
 class CkFftBatchDesc(ctypes.Structure):
@@ -118,6 +121,8 @@ _SIGS = {
     "CkFftServerResetStats": (None, [_vp]),
     "CkFftServerDestroy": (None, [_vp]),
     "CkFftServerLastError": (ctypes.c_char_p, []),
+    "CkFftSetPrecision": (_i, [_vp, _i]),
+    "CkFftGetPrecision": (_i, [_vp]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -209,6 +214,14 @@ class Context:
 
     def autotune(self, cache_path=None) -> int:
         return CkFftAutotune(self.handle, cache_path.encode() if cache_path else None)
+
+    def set_precision(self, precision: int) -> int:
+        """kCkFftPrecision_Fp32 (default) or kCkFftPrecision_SplitBf16x3 (complex
+        16 <= n <= 65536 on the matrix cores)."""
+        return CkFftSetPrecision(self.handle, precision)
+
+    def precision(self) -> int:
+        return CkFftGetPrecision(self.handle)
 
     def load_tuning(self, cache_path) -> int:
         return CkFftLoadTuning(self.handle, cache_path.encode())

@@ -93,7 +93,7 @@ int fail(hipError_t e, const char* where)
 
 DeviceTables tables(const CkFftContext* c)
 {
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision};
 }
 
 // One synchronous transform, as every drop-in call is.
@@ -207,6 +207,7 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->d_fine = fine;
     c->log_max = log_max;
     std::memset(c->tune, 0, sizeof(c->tune));
+    c->precision = kCkFftPrecision_Fp32;
     c->staging_mutex = mtx;
     c->d_staging = nullptr;
     c->staging_bytes = 0;
@@ -300,6 +301,20 @@ int CkFftGetTwiddles(CkFftContext* context, int L, CkFftComplex* out)
                   "copy twiddles")
                ? 1
                : 0;
+}
+
+int CkFftSetPrecision(CkFftContext* context, int precision)
+{
+    if (!context || context->magic != kContextMagic) return 0;
+    if (precision != kCkFftPrecision_Fp32 && precision != kCkFftPrecision_SplitBf16x3) return 0;
+    context->precision = precision;
+    return 1;
+}
+
+int CkFftGetPrecision(const CkFftContext* context)
+{
+    if (!context || context->magic != kContextMagic) return -1;
+    return context->precision;
 }
 
 const char* CkFftGpuLastError(void) { return g_last_error; }

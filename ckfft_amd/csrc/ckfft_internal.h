@@ -6,7 +6,7 @@
 #include <cstdint>
 #include <mutex>
 
-#include "ckfft/ckfft.h"
+#include "ckfft/ckfft_gpu.h"
 
 namespace ckfft_gpu {
 
@@ -43,6 +43,7 @@ struct DeviceTables {
     int log_pyramid;
     const float2* fine;      // null when log_max <= 16
     int log_max;             // largest log2(n) the tables serve
+    int precision;           // kCkFftPrecision_* of the context
 };
 
 // Launchers (return hipSuccess or an error).
@@ -57,6 +58,12 @@ hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, voi
 
 // Largest log2(n) launch_transform supports for a kind.
 int max_log2n(Kind kind);
+
+// Split-BF16x3 matrix-core path (fft_mfma.hip): complex kinds, 2^4 <= n <= 2^16.  The
+// dispatch above routes there when the tables' precision asks for it.
+bool mfma_supported(Kind kind, int log2n);
+hipError_t launch_mfma(Kind kind, int log2n, int count, const void* in, void* out,
+                       const DeviceTables& tables, hipStream_t stream);
 
 // Multi-pass path (fft_pass.hip) for sizes above one workgroup.
 hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, void* out,
@@ -78,6 +85,9 @@ struct _CkFftContext {
     // Tuned variant (points per thread 16 or 32) per [kind][log2 complex size], used by
     // batch plans; every variant has the same arithmetic.
     unsigned char tune[4][16];
+    // kCkFftPrecision_*: FP32 (default) or split-BF16x3 matrix-core radix-16 stages for
+    // the complex sizes that mode covers.
+    int precision;
     // Staging for host-pointer calls (grown on demand, guarded by staging_mutex).
     std::mutex* staging_mutex;
     void* d_staging;

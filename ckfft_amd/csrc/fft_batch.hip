@@ -243,7 +243,10 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         const int log2n = ilog2(d.n);
         const bool real = d.kind >= kRealForward;
         const int logc = real ? log2n - 1 : log2n;
-        const bool ok = real ? (log2n >= 3 && log2n <= kLogMaxLds + 1) : (log2n >= 1 && log2n <= kPassLogN);
+        bool ok = real ? (log2n >= 3 && log2n <= kLogMaxLds + 1) : (log2n >= 1 && log2n <= kPassLogN);
+        // the persistent kernel computes in FP32; transforms of the split-BF16x3 mode run
+        // as ordinary launches of that mode's kernels
+        if (ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n)) ok = false;
         if (ok)
             pers.push_back({d.kind, log2n, logc, i});
         else
@@ -314,7 +317,7 @@ hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const Ba
 
 DeviceTables context_tables(const CkFftContext* c)
 {
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max};
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision};
 }
 
 }  // namespace ckfft_gpu

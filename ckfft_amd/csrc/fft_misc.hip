@@ -176,6 +176,8 @@ hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, voi
     // complex size 2^logc; the real kinds run a half-size complex transform and read
     // the twiddles of level 2^log2n for their pack/unpack
     const int logc = real ? log2n - 1 : log2n;
+    if (tb.precision == kCkFftPrecision_SplitBf16x3 && mfma_supported(kind, log2n))
+        return launch_mfma(kind, log2n, count, in, out, tb, stream);
     if (logc <= kLogMaxLds) {
         if (tb.log_pyramid < log2n) return hipErrorInvalidValue;
         return launch_lds(kind, logc, count, in, out, tb.pyramid, stream);

@@ -144,6 +144,25 @@ void CkFftServerResetStats(CkFftServer* server);
 void CkFftServerDestroy(CkFftServer* server);
 const char* CkFftServerLastError(void);
 
+/* ---- precision mode ------------------------------------------------------------------
+ *
+ * kCkFftPrecision_Fp32 (default): every stage in FP32 on the vector units.
+ * kCkFftPrecision_SplitBf16x3 (opt-in): complex transforms with 16 <= n <= 65536 run
+ * their radix-16 stages as 32x32 DFT-matrix products on the matrix cores, each FP32
+ * operand split exactly into three BF16 parts, six partial products accumulated in FP32
+ * (error at the FP32 level; results differ in the last bits from the FP32 mode).  Real
+ * transforms and other sizes keep the FP32 kernels.  The mode applies to every later call
+ * on the context: single-signal, batched, batch plans created afterwards (those
+ * transforms then run as ordinary launches) and servers.  The reference has no
+ * counterpart.  Returns 1, or 0 for an invalid context or mode. */
+enum {
+    kCkFftPrecision_Fp32 = 0,
+    kCkFftPrecision_SplitBf16x3 = 1
+};
+int CkFftSetPrecision(CkFftContext* context, int precision);
+/* The context's mode, or -1 for an invalid context. */
+int CkFftGetPrecision(const CkFftContext* context);
+
 /* Copy the device twiddle table of level L (L values exp(-2 pi i k / L), k < L,
  * generated in FP64 on the GPU and rounded to FP32) to host memory `out`.
  * L must be a power of two no larger than the context's table.  Returns 1/0. */
