@@ -274,6 +274,89 @@ def serving_leg(torch, C, dev, rate: float = 200e3, seconds: float = 2.0, budget
     return out
 
 
+def event_ms(torch, stream, fn, reps: int, warmup: int = 2):
+    """Median duration in ms of `fn()` (one enqueue on `stream`) over `reps` event pairs."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+
+def complex_4096_leg(torch, C, O, dev, reps: int = 50):
+    """Complex forward then inverse round trip, N=4096, 1024 signals, re/im iid U[-1,1]
+    (SplitMix64 seed 0xC1): per-launch times, GB/s against 8 TB/s, errors of a sample."""
+    n, count = 4096, 1024
+    x_h = O.uniform_pm1(0xC1, count * 2 * n).reshape(count, 2 * n)
+    x = torch.from_numpy(x_h).to(dev)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    stream = torch.cuda.current_stream(dev)
+    with C.Context(n, C.kCkFftDirection_Both) as ctx:
+        fwd = event_ms(torch, stream, lambda: ctx.complex_forward_batch(n, count, x, y, stream), reps)
+        inv = event_ms(torch, stream, lambda: ctx.complex_inverse_batch(n, count, y, z, stream), reps)
+    torch.cuda.synchronize()
+    idx = [0, 511, 1023]
+    yh, zh = y[idx].cpu().numpy(), z[idx].cpu().numpy()
+    o = O.Restatement(n, 3)
+    nbytes = 16.0 * n * count
+    return {"workload": "complex fwd + inv, N=4096 x 1024, U[-1,1]",
+            "forward_ms": round(fwd, 5), "inverse_ms": round(inv, 5),
+            "GFLOP_s": round(2 * 5.0 * n * 12 * count / ((fwd + inv) * 1e-3) / 1e9, 1),
+            "forward_GBs": round(nbytes / (fwd * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(nbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "forward_vs_ref": float(f"{O.split_norm_err(yh, o.complex(x_h[idx], False, 8)):.3e}"),
+            "forward_relL2_fp64": float(f"{O.rel_l2(yh, O.interleave(O.fp64_complex(x_h[idx], False))):.3e}"),
+            "round_trip_relL2": float(f"{O.rel_l2(zh / n, x_h[idx]):.3e}")}
+
+
+def huge_leg(torch, C, O, dev, reps: int = 10):
+    """One complex forward transform of N=2^24 (128 MiB in, 128 MiB out), U[-1,1]: time,
+    GB/s against 8 TB/s (three global-memory passes move 3x the algorithmic bytes), and
+    the relative L2 error against the FP64 FFT."""
+    n = 1 << 24
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0xC3)
+    x = torch.rand(2 * n, device=dev, generator=gen) * 2 - 1
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream(dev)
+    with C.Context(n, C.kCkFftDirection_Forward) as ctx:
+        ms = event_ms(torch, stream, lambda: ctx.complex_forward_batch(n, 1, x, y, stream), reps)
+    torch.cuda.synchronize()
+    err = O.rel_l2(y.cpu().numpy(), O.interleave(O.fp64_complex(x.cpu().numpy(), False)))
+    gbs = 16.0 * n / (ms * 1e-3) / 1e9
+    return {"workload": "single complex forward, N=2^24, U[-1,1]", "ms": round(ms, 4),
+            "GFLOP_s": round(5.0 * n * 24 / (ms * 1e-3) / 1e9, 1), "achieved_GBs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "passes": 3,
+            "relL2_fp64": float(f"{err:.3e}")}
+
+
+def host_pointer_leg(C, O, signals: int = 16):
+    """PCIe-inclusive rate of the unmodified drop-in call: CkFftRealForward/Inverse on
+    host (numpy) buffers, n=65536, one synchronous call per signal (staged through device
+    memory inside the call)."""
+    n = N_REAL
+    x = O.normal01(0xB0, signals * n).reshape(signals, n)
+    y = np.empty((signals, 2 * (n // 2 + 1)), np.float32)
+    z = np.empty((signals, n), np.float32)
+    tmp = np.empty(2 * (n // 2 + 1), np.float32)
+    with C.Context(n, C.kCkFftDirection_Both) as ctx:
+        assert ctx.real_forward(n, x[0], y[0]) == 1
+        t0 = time.perf_counter()
+        for s in range(signals):
+            assert ctx.real_forward(n, x[s], y[s]) == 1
+            assert ctx.real_inverse(n, y[s], z[s], tmp) == 1
+        el = time.perf_counter() - t0
+    return {"workload": f"real fwd + inv, n=65536, {signals} signals, host buffers, one call each",
+            "value": round(signals * 2 * real_flops(n) / el / 1e9, 2), "unit": "GFLOP/s",
+            "us_per_call": round(el / (2 * signals) * 1e6, 1)}
+
+
 def split_bf16_leg(torch, C, O, dev, reps: int = 20, sizes=(4096, 8192, 16384, 32768, 65536),
                    in_bytes: int = 256 << 20):
     """Opt-in split-BF16x3 mode: complex forward transforms of N = 4096..65536, batched to
@@ -336,6 +419,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (persistent launch) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16x3 precision leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the complex 4096 x 1024, 2^24 and host-pointer legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
                          "HBM counters FETCH_SIZE / WRITE_SIZE")
@@ -456,6 +541,11 @@ def main():
     uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=2)
     serving = None if args.no_serve else serving_leg(torch, C, dev)
     split = None if args.no_split else split_bf16_leg(torch, C, O, dev)
+    configs = None
+    if not args.no_configs:
+        configs = {"complex_4096_x1024": complex_4096_leg(torch, C, O, dev),
+                   "complex_2p24": huge_leg(torch, C, O, dev),
+                   "host_pointers": host_pointer_leg(C, O)}
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
@@ -472,6 +562,7 @@ def main():
         "batched_uneven": uneven,
         "serving": serving,
         "split_bf16x3": split,
+        "other_configs": configs,
         "device": torch.cuda.get_device_name(dev),
     }
     print(json.dumps(line))

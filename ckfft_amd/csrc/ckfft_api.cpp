@@ -93,7 +93,7 @@ int fail(hipError_t e, const char* where)
 
 DeviceTables tables(const CkFftContext* c)
 {
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision};
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw};
 }
 
 // One synchronous transform, as every drop-in call is.
@@ -169,30 +169,36 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     if (!record(hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)), "hipMalloc(twiddles)")) return nullptr;
     const int log_max = std::min(ilog2(nMax), kLogMaxSupported);
     float2* fine = nullptr;
-    if (log_max > kLogMaxPyramid &&
-        !record(hipMalloc(&fine, sizeof(float2) * ((size_t)(log_max - kLogMaxPyramid) << kLogFine)),
-                "hipMalloc(fine twiddles)")) {
+    float2* xtw = nullptr;
+    const int log_xtw = log_pyr >= kLogMinXtw ? log_pyr : 0;
+    auto release = [&]() {
         (void)hipFree(pyr);
+        if (fine) (void)hipFree(fine);
+        if (xtw) (void)hipFree(xtw);
+    };
+    if ((log_max > kLogMaxPyramid &&
+         !record(hipMalloc(&fine, sizeof(float2) * ((size_t)(log_max - kLogMaxPyramid) << kLogFine)),
+                 "hipMalloc(fine twiddles)")) ||
+        (log_xtw && !record(hipMalloc(&xtw, sizeof(float2) << log_xtw), "hipMalloc(pass twiddles)"))) {
+        release();
         return nullptr;
     }
     if (!record(launch_twiddle_pyramid(pyr, log_pyr, nullptr), "twiddle launch") ||
         !record(launch_twiddle_fine(fine, log_max, nullptr), "fine twiddle launch") ||
+        !record(launch_twiddle_transposed(xtw, pyr, log_xtw, nullptr), "pass twiddle launch") ||
         !record(hipStreamSynchronize(nullptr), "twiddle synchronize")) {
-        (void)hipFree(pyr);
-        if (fine) (void)hipFree(fine);
+        release();
         return nullptr;
     }
 
     void* mem = buf ? buf : std::malloc(need);
     if (!mem) {
-        (void)hipFree(pyr);
-        if (fine) (void)hipFree(fine);
+        release();
         return nullptr;
     }
     std::mutex* mtx = new (std::nothrow) std::mutex();
     if (!mtx) {
-        (void)hipFree(pyr);
-        if (fine) (void)hipFree(fine);
+        release();
         if (!buf) std::free(mem);
         return nullptr;
     }
@@ -205,6 +211,8 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->log_pyramid = log_pyr;
     c->d_pyramid = pyr;
     c->d_fine = fine;
+    c->d_xtw = xtw;
+    c->log_xtw = log_xtw;
     c->log_max = log_max;
     std::memset(c->tune, 0, sizeof(c->tune));
     c->precision = kCkFftPrecision_Fp32;
@@ -257,6 +265,7 @@ void CkFftShutdown(CkFftContext* context)
     if (!context || context->magic != kContextMagic) return;
     (void)hipFree(context->d_pyramid);
     if (context->d_fine) (void)hipFree(context->d_fine);
+    if (context->d_xtw) (void)hipFree(context->d_xtw);
     if (context->d_staging) (void)hipFree(context->d_staging);
     delete context->staging_mutex;
     const int own = context->own_host;

@@ -28,6 +28,12 @@ constexpr int kLogFine = 12;
 // Largest supported transform (complex points or real samples): 2^28.  Twiddles of a
 // level L > 2^16 are a coarse pyramid entry (level L / 2^12 <= 2^16) times a fine one.
 constexpr int kLogMaxSupported = kLogMaxPyramid + kLogFine;
+// Transposed inter-pass twiddles: for the two-pass transforms whose second pass has span
+// 2^kLogXtwCols, table[r * 2^kLogXtwCols + j] = W_L^{r j} (L = 2^log_xtw, the context's
+// largest pyramid level when it is >= 2^kLogMinXtw), so that the columns j a tile reads
+// side by side find their twiddles side by side.
+constexpr int kLogXtwCols = 8;
+constexpr int kLogMinXtw = 15;
 // Radix range of one multi-pass stage.
 constexpr int kLogMinPass = 6;
 constexpr int kLogMaxPass = 12;
@@ -44,11 +50,14 @@ struct DeviceTables {
     const float2* fine;      // null when log_max <= 16
     int log_max;             // largest log2(n) the tables serve
     int precision;           // kCkFftPrecision_* of the context
+    const float2* xtw;       // transposed inter-pass twiddles (null when log_xtw == 0)
+    int log_xtw;
 };
 
 // Launchers (return hipSuccess or an error).
 hipError_t launch_twiddle_pyramid(float2* table, int log_lmax, hipStream_t stream);
 hipError_t launch_twiddle_fine(float2* table, int log_max, hipStream_t stream);
+hipError_t launch_twiddle_transposed(float2* table, const float2* pyramid, int log_l, hipStream_t stream);
 
 // Batched transform of `count` signals of size n (n >= 1, power of two, n <= the
 // kind's limit).  Pointers are device memory with the layouts of ckfft_gpu.h.  Sizes
@@ -81,6 +90,8 @@ struct _CkFftContext {
     int log_pyramid;         // levels 1..2^log_pyramid are on the device
     float2* d_pyramid;       // device twiddle pyramid, 2 << log_pyramid entries
     float2* d_fine;          // fine twiddle tables for levels above 2^16 (or null)
+    float2* d_xtw;           // transposed inter-pass twiddles (or null)
+    int log_xtw;             // their level, 0 when absent
     int log_max;             // largest log2(n) the tables serve
     // Tuned variant (points per thread 16 or 32) per [kind][log2 complex size], used by
     // batch plans; every variant has the same arithmetic.

@@ -102,14 +102,19 @@ __device__ __forceinline__ void run_twopass_item(const BatchItem& w, const Batch
                                                  const DeviceTables& tb, float* lds)
 {
     const BatchTransform d = trs[w.first];
+    // opaque thread index, as in run_lds_item: keeps this body's address arithmetic its own
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    __builtin_assume(tid >= 0 && tid < kPersistWG);
     for (int q = 0; q < kPassTiles; ++q) {
-        pass_tile<kPassLogR, kPassLogCB, INV, true>(d.in, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, lds,
-                                                     threadIdx.x);
+        pass_tile<kPassLogR, kPassLogCB, INV, true>(d.in, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, tb.xtw,
+                                                     tb.log_xtw, lds, tid);
         __syncthreads();
     }
     for (int q = 0; q < kPassTiles; ++q) {
-        pass_tile<kPassLogR, kPassLogCB, INV, false>(d.scratch, d.out, q, kPassLogN, kPassLogR, tb.pyramid, tb.fine,
-                                                      lds, threadIdx.x);
+        // a context that admits 2^16 transforms holds the transposed twiddles of level 2^16
+        pass_tile<kPassLogR, kPassLogCB, INV, false, 2>(d.scratch, d.out, q, kPassLogN, kPassLogR, tb.pyramid,
+                                                         tb.fine, tb.xtw, kPassLogN, lds, tid);
         __syncthreads();
     }
 }
@@ -317,7 +322,7 @@ hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const Ba
 
 DeviceTables context_tables(const CkFftContext* c)
 {
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision};
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw};
 }
 
 }  // namespace ckfft_gpu

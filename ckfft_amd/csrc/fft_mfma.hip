@@ -55,7 +55,8 @@ struct MfmaPassShape {
 // the inter-pass twiddle W_N^{r j} on load.  In and out may alias in the second pass.
 template <int LOGR, bool INV, bool FIRST>
 __global__ __launch_bounds__(512) void fft_mfma_pass_kernel(const float2* in, float2* out,
-                                                            const float2* __restrict__ pyr, int logn,
+                                                            const float2* __restrict__ pyr,
+                                                            const float2* __restrict__ xtw, int log_xtw, int logn,
                                                             long long tiles_per_signal, long long total_tiles)
 {
     using SH = MfmaPassShape<LOGR>;
@@ -75,7 +76,11 @@ __global__ __launch_bounds__(512) void fft_mfma_pass_kernel(const float2* in, fl
         const int r = e >> SH::LOGCB, b = e & (CB - 1);
         const long long j = j0 + b;
         float2 v = src[j + r * stride];
-        if constexpr (!FIRST) v = cmul(v, dirw<INV>(pyr[N + r * j]));
+        if constexpr (!FIRST) {
+            // W_N^{r j}: the transposed copy (columns side by side) when the context has it
+            const float2 w = xtw ? xtw[((long long)r << (log_xtw - logn)) * (1 << kLogXtwCols) + j] : pyr[N + r * j];
+            v = cmul(v, dirw<INV>(w));
+        }
         lds[tile_at<TL>(b, r)] = v;
     }
     __syncthreads();
@@ -122,7 +127,7 @@ static WholeFn whole_kernel(int logn, bool inv, int* wg, int* logb)
     }
 }
 
-using PassFnM = void (*)(const float2*, float2*, const float2*, int, long long, long long);
+using PassFnM = void (*)(const float2*, float2*, const float2*, const float2*, int, int, long long, long long);
 
 static PassFnM pass_kernel(int logr, bool inv, bool first)
 {
@@ -160,12 +165,15 @@ hipError_t launch_mfma(Kind kind, int log2n, int count, const void* in, void* ou
     const long long tiles1 = ((1ll << log2n) >> logr1) >> MfmaPassShape<kLogMfmaPassR1>::LOGCB;
     const long long tiles2 = ((1ll << log2n) >> logr2) >> (13 - logr2);
     if (tiles1 * count > 0x7fffffffll || tiles2 * count > 0x7fffffffll) return hipErrorInvalidValue;
+    // pass 2 has span 2^kLogMfmaPassR1 = 256 columns: the transposed twiddles apply
+    static_assert(kLogMfmaPassR1 == kLogXtwCols, "pass-2 span must match the transposed table");
+    const float2* xtw = tb.log_xtw >= log2n ? tb.xtw : nullptr;
     hipLaunchKernelGGL(p1, dim3((unsigned)(tiles1 * count)), dim3(512), 0, s, (const float2*)in, (float2*)out,
-                       tb.pyramid, log2n, tiles1, tiles1 * count);
+                       tb.pyramid, xtw, tb.log_xtw, log2n, tiles1, tiles1 * count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(p2, dim3((unsigned)(tiles2 * count)), dim3(512), 0, s, (const float2*)out, (float2*)out,
-                       tb.pyramid, log2n, tiles2, tiles2 * count);
+                       tb.pyramid, xtw, tb.log_xtw, log2n, tiles2, tiles2 * count);
     return hipGetLastError();
 }
 

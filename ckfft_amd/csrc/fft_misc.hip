@@ -67,6 +67,24 @@ hipError_t launch_twiddle_fine(float2* table, int log_max, hipStream_t stream)
     return hipGetLastError();
 }
 
+// Copies of pyramid entries (bit-identical values), laid out for the second pass of the
+// two-pass transforms: table[r * 256 + j] = pyramid level L entry r*j.
+__global__ void twiddle_transposed_kernel(float2* __restrict__ table, const float2* __restrict__ pyr, int log_l)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (1ll << log_l)) return;
+    const long long r = e >> kLogXtwCols, j = e & ((1 << kLogXtwCols) - 1);
+    table[e] = pyr[(1ll << log_l) + r * j];
+}
+
+hipError_t launch_twiddle_transposed(float2* table, const float2* pyramid, int log_l, hipStream_t stream)
+{
+    if (!table) return hipSuccess;
+    hipLaunchKernelGGL(twiddle_transposed_kernel, dim3((unsigned)(((1ll << log_l) + 255) / 256)), dim3(256), 0,
+                       stream, table, pyramid, log_l);
+    return hipGetLastError();
+}
+
 hipError_t launch_twiddle_pyramid(float2* table, int log_lmax, hipStream_t stream)
 {
     const long long total = 2ll << log_lmax;

@@ -66,10 +66,14 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
 // `lds` holds CB * RSTRIDE floats.  Used by the pass kernel below and by the persistent
 // kernel.
-template <int LOGR, int LOGCB, bool INV, bool FIRST>
+// XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
+// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time;
+// 2 = always from the transposed copy (the caller guarantees it covers the pass).
+template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
 __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
                                           int logn, int logns, const float2* __restrict__ pyr,
-                                          const float2* __restrict__ fine, float* lds, int tid)
+                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
+                                          int log_xtw, float* lds, int tid)
 {
     using SH = PassShape<LOGR, LOGCB>;
     using PL = Plan<LOGR, SH::LOGP>;
@@ -98,13 +102,27 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
     if constexpr (!FIRST) {
         const long long jm = (j0 + cA) & ((1ll << logns) - 1);
         const int loglev = logns + LOGR;
+        if (XTW == 2 || (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw)) {
+            // the same values from the transposed copy: neighbouring columns, neighbouring words
+            const float2* tcol = xtw + jm;
+            const int sh = log_xtw - loglev;
 #pragma unroll
-        for (int b = 0; b < B0; ++b)
+            for (int b = 0; b < B0; ++b)
 #pragma unroll
-            for (int rr = 0; rr < R0; ++rr) {
-                const int r = in_index<PL, 0>(tA + b * T, rr);
-                v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)r * jm)));
-            }
+                for (int rr = 0; rr < R0; ++rr) {
+                    const int r = in_index<PL, 0>(tA + b * T, rr);
+                    v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
+                }
+        } else if constexpr (XTW != 2) {
+#pragma unroll
+            for (int b = 0; b < B0; ++b)
+#pragma unroll
+                for (int rr = 0; rr < R0; ++rr) {
+                    const int r = in_index<PL, 0>(tA + b * T, rr);
+                    v[b * R0 + rr] =
+                        cmul(v[b * R0 + rr], dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)r * jm)));
+                }
+        }
     }
     stage_compute<PL, 0, INV>(v, tA, pyr);
     pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);

@@ -26,7 +26,8 @@ namespace ckfft_gpu {
 template <int LOGR, bool INV, bool FIRST>
 __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ pyr,
-    const float2* __restrict__ fine, int logn, int logns, long long tiles_per_signal, long long total_tiles)
+    const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, int logn, int logns,
+    long long tiles_per_signal, long long total_tiles)
 {
     using SH = PassShape<LOGR>;
     __shared__ float lds[SH::CB * SH::RSTRIDE];
@@ -34,7 +35,10 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const long long g = tile / tiles_per_signal;
     const long long q = tile - g * tiles_per_signal;
     const long long N = 1ll << logn;
-    pass_tile<LOGR, SH::LOGCB, INV, FIRST>(in + g * N, out + g * N, q, logn, logns, pyr, fine, lds, threadIdx.x);
+    // the transposed twiddles cover span 2^kLogXtwCols at levels <= 2^16: radix <= 2^8
+    constexpr int XTW = LOGR + kLogXtwCols <= kLogMaxPyramid ? 1 : 0;
+    pass_tile<LOGR, SH::LOGCB, INV, FIRST, XTW>(in + g * N, out + g * N, q, logn, logns, pyr, fine, xtw, log_xtw,
+                                                lds, threadIdx.x);
 }
 
 // ---- real pack / unpack for sizes above the one-workgroup limit ----------------------
@@ -81,7 +85,8 @@ __global__ __launch_bounds__(256) void real_unpack_kernel(const float2* __restri
 
 // ---- host side ----------------------------------------------------------------------
 
-using PassFn = void (*)(const float2*, float2*, const float2*, const float2*, int, int, long long, long long);
+using PassFn = void (*)(const float2*, float2*, const float2*, const float2*, const float2*, int, int, int, long long,
+                        long long);
 
 struct PassCfg {
     PassFn fn;
@@ -138,8 +143,8 @@ static hipError_t launch_pass(int logr, bool inv, bool first, const float2* in, 
     const long long tiles = ((1ll << logc) >> logr) / c.cb;
     const long long total = tiles * count;
     if (total > 0x7fffffffll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(c.fn, dim3((unsigned)total), dim3(c.wg), 0, s, in, out, tb.pyramid, tb.fine, logc, logns,
-                       tiles, total);
+    hipLaunchKernelGGL(c.fn, dim3((unsigned)total), dim3(c.wg), 0, s, in, out, tb.pyramid, tb.fine, tb.xtw, tb.log_xtw,
+                       logc, logns, tiles, total);
     return hipGetLastError();
 }
 

@@ -184,9 +184,15 @@ __device__ __forceinline__ void exchange(float2* v, int t, float* re, float* im)
         __syncthreads();
         lds_read<PL, SR, 1, MR>(v, t, re, im);
         __syncthreads();
-        lds_write<PL, SW, 2, MW>(v, t, re, im);
+        // The imaginary plane recomputes its LDS addresses from an opaque copy of t.
+        // Otherwise the compiler keeps every address of the real-plane pass live across
+        // the barriers; mirror-closed stages have one address register per point, and
+        // at 64 points per thread that spills.
+        int t2 = t;
+        asm volatile("" : "+v"(t2));
+        lds_write<PL, SW, 2, MW>(v, t2, re, im);
         __syncthreads();
-        lds_read<PL, SR, 2, MR>(v, t, re, im);
+        lds_read<PL, SR, 2, MR>(v, t2, re, im);
         __syncthreads();
     }
 }
