@@ -340,7 +340,7 @@ def ctx_big(torch, C):
 @pytest.mark.parametrize("kind", ["cfwd", "cinv"])
 @pytest.mark.parametrize("k", range(16, 26))
 def test_complex_multipass_sizes(torch, ctx_big, kind, k):
-    """2 passes for 2^16..2^24 (every radix split 2^8 x 2^8 .. 2^12 x 2^12), 3 above."""
+    """2 passes for 2^16..2^22 (radix splits 2^8 x 2^8 .. 2^11 x 2^11), 3 passes from 2^23."""
     n = 1 << k
     count = 2 if k <= 20 else 1
     x = make_input(kind, n, count, 8000 + 10 * k)
