@@ -473,18 +473,26 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    # whole-step timing: K steps between two events, synchronised on both sides
+    # whole-step timing: K steps back to back between two events, synchronised on both
+    # sides (no events between the launches: they would add a gap to every launch)
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start.record(stream)
     for k in range(args.steps):
-        step(evs[k])
+        step()
     t_end.record(stream)
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
     ms_per_step = total_ms / args.steps
+    # per-launch durations for the roofline: the same K steps again, each launch between
+    # its own pair of events on the launch stream (the events add a small gap, so these
+    # durations are upper bounds of the kernels' time in the step above)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
     fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     inv_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
 
@@ -502,6 +510,8 @@ def main():
         "traffic": (traffic or {}).get(dom_name),
         "algorithmic_bytes_per_launch": bytes_launch,
         "launch_ms": {"real_forward": round(fwd_ms, 5), "real_inverse": round(inv_ms, 5)},
+        "launch_timing": "mean over K event-bracketed launches (a second pass of the K steps)",
+        "step_GBs": round(2 * bytes_launch / (ms_per_step * 1e-3) / 1e9, 1),
         "frac_per_launch": {"real_forward": round(bytes_launch / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "real_inverse": round(bytes_launch / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }

@@ -29,6 +29,7 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    "-fconstexpr-steps=200000000",   # compile-time checks of the LDS index split
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 

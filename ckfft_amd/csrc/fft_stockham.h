@@ -139,35 +139,129 @@ __device__ __forceinline__ int out_index(int j, int r)
     return (j / NS) * NS * R + (j & (NS - 1)) + r * NS;
 }
 
+// ---- LDS addresses: one per-thread base plus compile-time offsets ----------------
+//
+// Every exchange index here -- in_index / out_index of butterfly bfly(t, b), element r --
+// splits as U(t, g) + C(b, r): U depends on the thread and on the butterfly's group g
+// (0: j = t + bT; in a mirror-closed stage 1: the first mirrored butterfly, where thread
+// 0 takes M/2, and 2: the other mirrored ones), C is a compile-time constant.  The low
+// four bits of U and C never carry into each other, so the padded index is
+// pad(U) + pad(C): each thread computes at most three bases once and every access is an
+// immediate offset.  ok() checks the identity for every t, b and r at compile time; the
+// stages where it fails (transforms of fewer than 16 threads) use the plain formula.
+
+__host__ __device__ constexpr int pad_any(int x) { return x + (x >> 4); }   // lds_pad, any sign
+
+template <class PL, int S, bool MIRROR, bool OUT>
+struct IndexSplit {
+    static constexpr int R = 1 << PL::rlog(S);
+    static constexpr int NS = 1 << PL::nslog(S);
+    static constexpr int B = PL::P / R;
+    static constexpr int M = PL::N >> PL::rlog(S);
+    static constexpr int T = PL::T;
+    static_assert(!(MIRROR && OUT) || NS == 1, "mirror-closed stages are edge stages");
+
+    static constexpr int group(int b) { return (!MIRROR || b < B / 2) ? 0 : (b == B / 2 ? 1 : 2); }
+    static constexpr int butterfly(int t, int b)
+    {
+        if (group(b) == 0) return t + b * T;
+        const int jl = t + (b - B / 2) * T;
+        return jl == 0 ? M / 2 : M - jl;
+    }
+    static constexpr int reference(int t, int b, int r)
+    {
+        const int j = butterfly(t, b);
+        return OUT ? (j / NS) * NS * R + (j & (NS - 1)) + r * NS : j + r * M;
+    }
+    static constexpr int U(int t, int g)
+    {
+        const int scale = OUT ? R : 1;                       // out index of span 1 is j*R + r
+        if (g == 1) return (t == 0 ? -(M / 2) : -t) * scale;
+        if (g == 2) return -t * scale;
+        if (!OUT) return t;
+        return NS >= T ? t : (t / NS) * NS * R + (t & (NS - 1));
+    }
+    static constexpr int C(int b, int r)
+    {
+        const int g = group(b);
+        if (!OUT) return g == 0 ? b * T + r * M : M - (b - B / 2) * T + r * M;
+        if (g != 0) return (M - (b - B / 2) * T) * R + r;
+        const int bt = b * T;
+        return NS >= T ? (bt / NS) * NS * R + (bt % NS) + r * NS : (bt / NS) * NS * R + r * NS;
+    }
+    static constexpr bool ok()
+    {
+        for (int t = 0; t < T; ++t)
+            for (int b = 0; b < B; ++b)
+                for (int r = 0; r < R; ++r) {
+                    const int ref = reference(t, b, r), u = U(t, group(b)), c = C(b, r);
+                    if (u + c != ref || pad_any(u) + pad_any(c) != pad_any(ref)) return false;
+                }
+        return true;
+    }
+};
+
 // COMP: 1 = real plane, 2 = imaginary plane, 3 = both.
 template <class PL, int S, int COMP, bool MIRROR = false>
 __device__ __forceinline__ void lds_write(const float2* v, int t, float* re, float* im)
 {
-    constexpr int R = 1 << PL::rlog(S);
-    constexpr int B = PL::P / R;
+    using SP = IndexSplit<PL, S, MIRROR, true>;
+    constexpr int R = SP::R, B = SP::B;
+    if constexpr (SP::ok()) {
+        const int base0 = pad_any(SP::U(t, 0));
+        const int base1 = MIRROR ? pad_any(SP::U(t, 1)) : 0;
+        const int base2 = MIRROR ? pad_any(SP::U(t, 2)) : 0;
 #pragma unroll
-    for (int b = 0; b < B; ++b)
+        for (int b = 0; b < B; ++b) {
+            const int base = SP::group(b) == 0 ? base0 : (SP::group(b) == 1 ? base1 : base2);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = lds_pad(out_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
-            if constexpr (COMP & 1) re[i] = v[b * R + r].x;
-            if constexpr (COMP & 2) im[i] = v[b * R + r].y;
+            for (int r = 0; r < R; ++r) {
+                const int i = base + pad_any(SP::C(b, r));
+                if constexpr (COMP & 1) re[i] = v[b * R + r].x;
+                if constexpr (COMP & 2) im[i] = v[b * R + r].y;
+            }
         }
+    } else {   // small transforms (fewer than 16 threads) where the split carries
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lds_pad(out_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+                if constexpr (COMP & 1) re[i] = v[b * R + r].x;
+                if constexpr (COMP & 2) im[i] = v[b * R + r].y;
+            }
+    }
 }
 
 template <class PL, int S, int COMP, bool MIRROR = false>
 __device__ __forceinline__ void lds_read(float2* v, int t, const float* re, const float* im)
 {
-    constexpr int R = 1 << PL::rlog(S);
-    constexpr int B = PL::P / R;
+    using SP = IndexSplit<PL, S, MIRROR, false>;
+    constexpr int R = SP::R, B = SP::B;
+    if constexpr (SP::ok()) {
+        const int base0 = pad_any(SP::U(t, 0));
+        const int base1 = MIRROR ? pad_any(SP::U(t, 1)) : 0;
+        const int base2 = MIRROR ? pad_any(SP::U(t, 2)) : 0;
 #pragma unroll
-    for (int b = 0; b < B; ++b)
+        for (int b = 0; b < B; ++b) {
+            const int base = SP::group(b) == 0 ? base0 : (SP::group(b) == 1 ? base1 : base2);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = lds_pad(in_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
-            if constexpr (COMP & 1) v[b * R + r].x = re[i];
-            if constexpr (COMP & 2) v[b * R + r].y = im[i];
+            for (int r = 0; r < R; ++r) {
+                const int i = base + pad_any(SP::C(b, r));
+                if constexpr (COMP & 1) v[b * R + r].x = re[i];
+                if constexpr (COMP & 2) v[b * R + r].y = im[i];
+            }
         }
+    } else {   // small transforms (fewer than 16 threads) where the split carries
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lds_pad(in_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+                if constexpr (COMP & 1) v[b * R + r].x = re[i];
+                if constexpr (COMP & 2) v[b * R + r].y = im[i];
+            }
+    }
 }
 
 // Move data from the output layout of stage SW to the input layout of stage SR.
