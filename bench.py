@@ -336,25 +336,35 @@ def huge_leg(torch, C, O, dev, reps: int = 10):
             "relL2_fp64": float(f"{err:.3e}")}
 
 
-def host_pointer_leg(C, O, signals: int = 16):
+def host_pointer_leg(C, O, signals: int = 32):
     """PCIe-inclusive rate of the unmodified drop-in call: CkFftRealForward/Inverse on
-    host (numpy) buffers, n=65536, one synchronous call per signal (staged through device
-    memory inside the call)."""
+    pageable host (numpy) buffers, n=65536, one synchronous call per signal (staged through
+    pinned bounce buffers inside the call), from one thread and from 4 threads sharing the
+    context (the reference allows concurrent use of a context)."""
     n = N_REAL
     x = O.normal01(0xB0, signals * n).reshape(signals, n)
-    y = np.empty((signals, 2 * (n // 2 + 1)), np.float32)
-    z = np.empty((signals, n), np.float32)
-    tmp = np.empty(2 * (n // 2 + 1), np.float32)
+    y = np.zeros((signals, 2 * (n // 2 + 1)), np.float32)   # touched: no first-write page faults timed
+    z = np.zeros((signals, n), np.float32)
+    out = {"workload": f"real fwd + inv, n=65536, {signals} signals, pageable host buffers, one call each"}
     with C.Context(n, C.kCkFftDirection_Both) as ctx:
-        assert ctx.real_forward(n, x[0], y[0]) == 1
-        t0 = time.perf_counter()
-        for s in range(signals):
-            assert ctx.real_forward(n, x[s], y[s]) == 1
-            assert ctx.real_inverse(n, y[s], z[s], tmp) == 1
-        el = time.perf_counter() - t0
-    return {"workload": f"real fwd + inv, n=65536, {signals} signals, host buffers, one call each",
-            "value": round(signals * 2 * real_flops(n) / el / 1e9, 2), "unit": "GFLOP/s",
-            "us_per_call": round(el / (2 * signals) * 1e6, 1)}
+        def work(lo, hi):
+            tmp = np.empty(2 * (n // 2 + 1), np.float32)
+            for s in range(lo, hi):
+                assert ctx.real_forward(n, x[s], y[s]) == 1
+                assert ctx.real_inverse(n, y[s], z[s], tmp) == 1
+        work(0, 2)
+        for threads in (1, 4):
+            per = signals // threads
+            ts = [threading.Thread(target=work, args=(i * per, (i + 1) * per)) for i in range(threads)]
+            t0 = time.perf_counter()
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+            el = time.perf_counter() - t0
+            out[f"threads_{threads}"] = {"value": round(signals * 2 * real_flops(n) / el / 1e9, 2),
+                                         "unit": "GFLOP/s", "us_per_call": round(el / (2 * signals) * 1e6, 1)}
+    return out
 
 
 def split_bf16_leg(torch, C, O, dev, reps: int = 20, sizes=(4096, 8192, 16384, 32768, 65536),

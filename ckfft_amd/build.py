@@ -22,7 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libckfft_amd.so")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
-SOURCES = ["fft_lds.hip", "fft_pass.hip", "fft_batch.hip", "fft_mfma.hip", "fft_misc.hip", "ckfft_api.cpp", "fft_serve.cpp"]
+SOURCES = ["fft_lds.hip", "fft_pass.hip", "fft_batch.hip", "fft_mfma.hip", "fft_misc.hip", "ckfft_api.cpp", "fft_serve.cpp", "fft_staging.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = [

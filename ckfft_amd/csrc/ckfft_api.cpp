@@ -96,37 +96,16 @@ DeviceTables tables(const CkFftContext* c)
     return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw};
 }
 
-// One synchronous transform, as every drop-in call is.
+// One synchronous transform, as every drop-in call is: device buffers in place on the
+// null stream, host buffers through a staging slot.
 int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
 {
     const int log2n = ilog2(n);
     if (log2n > max_log2n(kind)) return 0;
-    const bool din = is_device_pointer(in), dout = is_device_pointer(out);
-    hipStream_t s = nullptr;
-    if (din && dout) {
-        hipError_t e = launch_transform(kind, log2n, 1, in, out, tables(c), s);
-        if (e != hipSuccess) return fail(e, "launch");
-        return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
-    }
-    // Host buffers: stage through the context's device buffer.
-    const size_t ib = in_bytes(kind, n), ob = out_bytes(kind, n);
-    const size_t ib_al = (ib + 255) & ~size_t(255);
-    std::lock_guard<std::mutex> lock(*c->staging_mutex);
-    const size_t need = ib_al + ob;
-    if (c->staging_bytes < need) {
-        if (c->d_staging) (void)hipFree(c->d_staging);
-        c->d_staging = nullptr;
-        c->staging_bytes = 0;
-        if (!record(hipMalloc(&c->d_staging, need), "hipMalloc(staging)")) return 0;
-        c->staging_bytes = need;
-    }
-    char* st = (char*)c->d_staging;
-    const void* src = din ? in : (const void*)st;
-    void* dst = dout ? out : (void*)(st + ib_al);
-    if (!din && !record(hipMemcpyAsync(st, in, ib, hipMemcpyHostToDevice, s), "copy in")) return 0;
-    if (!record(launch_transform(kind, log2n, 1, src, dst, tables(c), s), "launch")) return 0;
-    if (!dout && !record(hipMemcpyAsync(out, dst, ob, hipMemcpyDeviceToHost, s), "copy out")) return 0;
-    return record(hipStreamSynchronize(s), "synchronize") ? 1 : 0;
+    return record(staged_transform(c->staging, kind, log2n, in, in_bytes(kind, n), out, out_bytes(kind, n), tables(c)),
+                  "transform")
+               ? 1
+               : 0;
 }
 
 int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void* out, void* stream)
@@ -196,8 +175,8 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
         release();
         return nullptr;
     }
-    std::mutex* mtx = new (std::nothrow) std::mutex();
-    if (!mtx) {
+    StagingPool* pool = staging_create();
+    if (!pool) {
         release();
         if (!buf) std::free(mem);
         return nullptr;
@@ -216,9 +195,7 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->log_max = log_max;
     std::memset(c->tune, 0, sizeof(c->tune));
     c->precision = kCkFftPrecision_Fp32;
-    c->staging_mutex = mtx;
-    c->d_staging = nullptr;
-    c->staging_bytes = 0;
+    c->staging = pool;
     return c;
 }
 
@@ -266,8 +243,7 @@ void CkFftShutdown(CkFftContext* context)
     (void)hipFree(context->d_pyramid);
     if (context->d_fine) (void)hipFree(context->d_fine);
     if (context->d_xtw) (void)hipFree(context->d_xtw);
-    if (context->d_staging) (void)hipFree(context->d_staging);
-    delete context->staging_mutex;
+    staging_destroy(context->staging);
     const int own = context->own_host;
     context->magic = 0;
     context->~CkFftContext();

@@ -74,6 +74,15 @@ bool mfma_supported(Kind kind, int log2n);
 hipError_t launch_mfma(Kind kind, int log2n, int count, const void* in, void* out,
                        const DeviceTables& tables, hipStream_t stream);
 
+// Host-pointer staging (fft_staging.cpp): a pool of slots (stream, device buffer, pinned
+// bounce buffer) per context; one synchronous transform with any mix of host and device
+// buffers, pipelining the copies.
+struct StagingPool;
+StagingPool* staging_create();
+void staging_destroy(StagingPool* pool);
+hipError_t staged_transform(StagingPool* pool, Kind kind, int log2n, const void* in, size_t in_bytes, void* out,
+                            size_t out_bytes, const DeviceTables& tables);
+
 // Multi-pass path (fft_pass.hip) for sizes above one workgroup.
 hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, void* out,
                             const DeviceTables& tables, hipStream_t stream);
@@ -99,10 +108,8 @@ struct _CkFftContext {
     // kCkFftPrecision_*: FP32 (default) or split-BF16x3 matrix-core radix-16 stages for
     // the complex sizes that mode covers.
     int precision;
-    // Staging for host-pointer calls (grown on demand, guarded by staging_mutex).
-    std::mutex* staging_mutex;
-    void* d_staging;
-    size_t staging_bytes;
+    // Staging slots for host-pointer calls.
+    ckfft_gpu::StagingPool* staging;
 };
 
 constexpr uint64_t kContextMagic = 0x434b46465447505bull;  // "CKFFTGP["

@@ -35,6 +35,14 @@ constexpr int kPersistWG = 512;       // threads per persistent workgroup
 constexpr int kPassLogN = 16;         // complex size run as two passes inside one workgroup
 constexpr int kCtlWords = 2;          // control words of one launch: queue head, finished count
 
+// A queue small enough to travel in the kernel arguments (no upload before the launch).
+constexpr int kInlineQueue = 32;
+struct InlineQueue {
+    BatchItem items[kInlineQueue];
+    BatchTransform trs[kInlineQueue];
+    int n_items;
+};
+
 // Host description of a batch.
 struct BatchBuild {
     std::vector<BatchItem> items;
@@ -55,6 +63,9 @@ void batch_assign_scratch(BatchBuild& b, float2* scratch);
 int persistent_grid(const CkFftContext* ctx);
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
                                  int grid, const DeviceTables& tb, hipStream_t s);
+// The same launch with the queue of `b` (scratch assigned) passed by value; requires at
+// most kInlineQueue items and transforms.
+hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, const DeviceTables& tb, hipStream_t s);
 DeviceTables context_tables(const CkFftContext* c);
 long long transform_bytes(int kind, int log2n);
 bool is_device_memory(const void* p);

@@ -106,25 +106,44 @@ __device__ __forceinline__ void run_twopass_item(const BatchItem& w, const Batch
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     __builtin_assume(tid >= 0 && tid < kPersistWG);
-    for (int q = 0; q < kPassTiles; ++q) {
-        pass_tile<kPassLogR, kPassLogCB, INV, true>(d.in, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, tb.xtw,
-                                                     tb.log_xtw, lds, tid);
+    using PT = PassTile<kPassLogR, kPassLogCB>;
+    static_assert(kPassTiles % 2 == 0, "tiles run in pairs");
+    // Each pass loads tile q+1 into the other register set while tile q computes.
+    float2 va[PT::P], vb[PT::P];
+    pass_tile_load<kPassLogR, kPassLogCB>(d.in, 0, kPassLogN, tid, va);
+#pragma unroll 1
+    for (int q = 0; q < kPassTiles; q += 2) {
+        pass_tile_load<kPassLogR, kPassLogCB>(d.in, q + 1, kPassLogN, tid, vb);
+        pass_tile_run<kPassLogR, kPassLogCB, INV, true>(va, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, tb.xtw,
+                                                        tb.log_xtw, lds, tid);
+        __syncthreads();
+        if (q + 2 < kPassTiles) pass_tile_load<kPassLogR, kPassLogCB>(d.in, q + 2, kPassLogN, tid, va);
+        pass_tile_run<kPassLogR, kPassLogCB, INV, true>(vb, d.scratch, q + 1, kPassLogN, 0, tb.pyramid, tb.fine,
+                                                        tb.xtw, tb.log_xtw, lds, tid);
         __syncthreads();
     }
-    for (int q = 0; q < kPassTiles; ++q) {
-        // a context that admits 2^16 transforms holds the transposed twiddles of level 2^16
-        pass_tile<kPassLogR, kPassLogCB, INV, false, 2>(d.scratch, d.out, q, kPassLogN, kPassLogR, tb.pyramid,
-                                                         tb.fine, tb.xtw, kPassLogN, lds, tid);
+    // Pass 2 reads what pass 1 wrote: its first load follows the barrier that ends pass 1.
+    // A context that admits 2^16 transforms holds the transposed twiddles of level 2^16.
+    pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, 0, kPassLogN, tid, va);
+#pragma unroll 1
+    for (int q = 0; q < kPassTiles; q += 2) {
+        pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, q + 1, kPassLogN, tid, vb);
+        pass_tile_run<kPassLogR, kPassLogCB, INV, false, 2>(va, d.out, q, kPassLogN, kPassLogR, tb.pyramid, tb.fine,
+                                                            tb.xtw, kPassLogN, lds, tid);
+        __syncthreads();
+        if (q + 2 < kPassTiles) pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, q + 2, kPassLogN, tid, va);
+        pass_tile_run<kPassLogR, kPassLogCB, INV, false, 2>(vb, d.out, q + 1, kPassLogN, kPassLogR, tb.pyramid,
+                                                            tb.fine, tb.xtw, kPassLogN, lds, tid);
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchItem* __restrict__ items, int n_items,
-                                                                     const BatchTransform* __restrict__ trs,
-                                                                     int* __restrict__ ctl, DeviceTables tb)
+// The persistent loop, shared by the kernel that reads its queue from device memory and
+// the one that receives a small queue by value in its arguments.
+__device__ __forceinline__ void persistent_body(const BatchItem* __restrict__ items, int n_items,
+                                                const BatchTransform* __restrict__ trs, int* __restrict__ ctl,
+                                                const DeviceTables& tb, float* lds, int& s_item)
 {
-    __shared__ float lds[kPersistLdsFloats];
-    __shared__ int s_item;
     const int tid = threadIdx.x;
     if (tid == 0) s_item = atomicAdd(ctl, 1);
     for (;;) {
@@ -170,6 +189,24 @@ __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchI
             atomicExch(ctl + 1, 0);
         }
     }
+}
+
+__global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchItem* __restrict__ items, int n_items,
+                                                                     const BatchTransform* __restrict__ trs,
+                                                                     int* __restrict__ ctl, DeviceTables tb)
+{
+    __shared__ float lds[kPersistLdsFloats];
+    __shared__ int s_item;
+    persistent_body(items, n_items, trs, ctl, tb, lds, s_item);
+}
+
+// Same, with the queue in the kernel arguments: a launch that needs no upload.
+__global__ __launch_bounds__(kPersistWG) void fft_persistent_inline_kernel(InlineQueue q, int* __restrict__ ctl,
+                                                                            DeviceTables tb)
+{
+    __shared__ float lds[kPersistLdsFloats];
+    __shared__ int s_item;
+    persistent_body(q.items, q.n_items, q.trs, ctl, tb, lds, s_item);
 }
 
 }  // namespace ckfft_gpu
@@ -311,6 +348,17 @@ int persistent_grid(const CkFftContext* ctx)
         !batch_error(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count"))
         return 0;
     return std::max(1, per_cu * cus);
+}
+
+hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, const DeviceTables& tb, hipStream_t s)
+{
+    if (b.items.size() > (size_t)kInlineQueue || b.trs.size() > (size_t)kInlineQueue) return hipErrorInvalidValue;
+    InlineQueue q{};
+    for (size_t i = 0; i < b.items.size(); ++i) q.items[i] = b.items[i];
+    for (size_t i = 0; i < b.trs.size(); ++i) q.trs[i] = b.trs[i];
+    q.n_items = (int)b.items.size();
+    hipLaunchKernelGGL(fft_persistent_inline_kernel, dim3(grid), dim3(kPersistWG), 0, s, q, d_ctl, tb);
+    return hipGetLastError();
 }
 
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,

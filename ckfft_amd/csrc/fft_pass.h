@@ -65,22 +65,51 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
 // One tile of one pass: butterflies q*CB .. q*CB + CB-1 of the signal at `src` (2^logn
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
 // `lds` holds CB * RSTRIDE floats.  Used by the pass kernel below and by the persistent
-// kernel.
+// kernel.  A tile is split into its loads (pass_tile_load: the tile's points into the
+// thread's registers) and the rest (pass_tile_run: inter-pass twiddles, the sub-stages,
+// the stores), so that a caller running several tiles can load the next one while it
+// computes the current one.
 // XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
 // when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time;
 // 2 = always from the transposed copy (the caller guarantees it covers the pass).
-template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
-__device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
-                                          int logn, int logns, const float2* __restrict__ pyr,
-                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
-                                          int log_xtw, float* lds, int tid)
-{
+template <int LOGR, int LOGCB>
+struct PassTile {
     using SH = PassShape<LOGR, LOGCB>;
     using PL = Plan<LOGR, SH::LOGP>;
     static_assert(PL::NSTAGES >= 2, "a pass needs at least two sub-stages");
+    static constexpr int P = PL::P;
+};
+
+template <int LOGR, int LOGCB>
+__device__ __forceinline__ void pass_tile_load(const float2* __restrict__ src, long long q, int logn, int tid,
+                                               float2* v)
+{
+    using SH = PassShape<LOGR, LOGCB>;
+    using PL = typename PassTile<LOGR, LOGCB>::PL;
+    constexpr int T = SH::T, CB = SH::CB;
+    const long long stride = (1ll << logn) >> LOGR;          // N/R
+    const long long j0 = q * CB;
+    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;   // DFT-fastest lanes
+    constexpr int R0 = 1 << PL::rlog(0);
+    constexpr int B0 = PL::P / R0;
+#pragma unroll
+    for (int b = 0; b < B0; ++b)
+#pragma unroll
+        for (int rr = 0; rr < R0; ++rr) {
+            const int r = in_index<PL, 0>(tA + b * T, rr);
+            v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
+        }
+}
+
+template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
+__device__ __forceinline__ void pass_tile_run(float2* v, float2* __restrict__ dst,
+                                              long long q, int logn, int logns, const float2* __restrict__ pyr,
+                                              const float2* __restrict__ fine, const float2* __restrict__ xtw,
+                                              int log_xtw, float* lds, int tid)
+{
+    using SH = PassShape<LOGR, LOGCB>;
+    using PL = typename PassTile<LOGR, LOGCB>::PL;
     constexpr int R = SH::R, T = SH::T, CB = SH::CB;
-    const long long N = 1ll << logn;
-    const long long stride = N >> LOGR;                     // N/R
     const long long j0 = q * CB;
 
     const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
@@ -91,14 +120,6 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
     float* imB = reB + SH::SPAN;
     constexpr int R0 = 1 << PL::rlog(0);
     constexpr int B0 = PL::P / R0;
-    float2 v[PL::P];
-#pragma unroll
-    for (int b = 0; b < B0; ++b)
-#pragma unroll
-        for (int rr = 0; rr < R0; ++rr) {
-            const int r = in_index<PL, 0>(tA + b * T, rr);
-            v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
-        }
     if constexpr (!FIRST) {
         const long long jm = (j0 + cA) & ((1ll << logns) - 1);
         const int loglev = logns + LOGR;
@@ -147,6 +168,17 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
             for (int rr = 0; rr < RL; ++rr)
                 base[(long long)out_index<PL, SL>(tA + b * T, rr) * ns] = v[b * RL + rr];
     }
+}
+
+template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
+__device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
+                                          int logn, int logns, const float2* __restrict__ pyr,
+                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
+                                          int log_xtw, float* lds, int tid)
+{
+    float2 v[PassTile<LOGR, LOGCB>::P];
+    pass_tile_load<LOGR, LOGCB>(src, q, logn, tid, v);
+    pass_tile_run<LOGR, LOGCB, INV, FIRST, XTW>(v, dst, q, logn, logns, pyr, fine, xtw, log_xtw, lds, tid);
 }
 
 }  // namespace ckfft_gpu

@@ -146,7 +146,11 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
     batch_assign_scratch(b, sl.d_scratch);
     const DeviceTables tb = context_tables(S->ctx);
     bool ok = true;
-    if (!b.items.empty()) {
+    if (!b.items.empty() && b.items.size() <= (size_t)kInlineQueue && b.trs.size() <= (size_t)kInlineQueue) {
+        // small batch: the queue travels in the launch arguments
+        ok = serr(launch_persistent_inline(b, sl.d_ctl, std::min(S->grid, (int)b.items.size()), tb, sl.stream),
+                  "persistent launch");
+    } else if (!b.items.empty()) {
         const size_t items_bytes = sizeof(BatchItem) * b.items.size();
         const size_t trs_off = (items_bytes + 255) & ~(size_t)255;
         const size_t total = trs_off + sizeof(BatchTransform) * b.trs.size();

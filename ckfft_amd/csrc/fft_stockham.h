@@ -147,8 +147,8 @@ __device__ __forceinline__ int out_index(int j, int r)
 // 0 takes M/2, and 2: the other mirrored ones), C is a compile-time constant.  The low
 // four bits of U and C never carry into each other, so the padded index is
 // pad(U) + pad(C): each thread computes at most three bases once and every access is an
-// immediate offset.  ok() checks the identity for every t, b and r at compile time; the
-// stages where it fails (transforms of fewer than 16 threads) use the plain formula.
+// immediate offset.  ok() proves the split at compile time; the stages where it cannot
+// (transforms of fewer than 16 threads) use the plain formula.
 
 __host__ __device__ constexpr int pad_any(int x) { return x + (x >> 4); }   // lds_pad, any sign
 
@@ -189,16 +189,57 @@ struct IndexSplit {
         const int bt = b * T;
         return NS >= T ? (bt / NS) * NS * R + (bt % NS) + r * NS : (bt / NS) * NS * R + r * NS;
     }
+    // U + C equals the index by construction (T and NS are powers of two, one divides the
+    // other; a sample of threads re-checks it).  pad(U + C) = pad(U) + pad(C) exactly when
+    // (U & 15) + (C & 15) < 16 (floor division by 16 on both sides), so the bound below,
+    // the largest low part of U over all threads plus that of C over the group's
+    // elements, makes the split exact for every t, b and r.
     static constexpr bool ok()
     {
-        for (int t = 0; t < T; ++t)
+        const int probe[6] = {0, 1, 15, 16, T / 2, T - 1};
+        for (int k = 0; k < 6; ++k) {
+            const int t = probe[k] < T ? probe[k] : T - 1;
             for (int b = 0; b < B; ++b)
-                for (int r = 0; r < R; ++r) {
-                    const int ref = reference(t, b, r), u = U(t, group(b)), c = C(b, r);
-                    if (u + c != ref || pad_any(u) + pad_any(c) != pad_any(ref)) return false;
-                }
+                for (int r = 0; r < R; ++r)
+                    if (U(t, group(b)) + C(b, r) != reference(t, b, r)) return false;
+        }
+        for (int g = 0; g < 3; ++g) {
+            int mu = -1, mc = -1;
+            for (int b = 0; b < B; ++b)
+                if (group(b) == g)
+                    for (int r = 0; r < R; ++r) mc = (C(b, r) & 15) > mc ? (C(b, r) & 15) : mc;
+            if (mc < 0) continue;                                   // no butterfly in this group
+            for (int t = 0; t < T; ++t) mu = (U(t, g) & 15) > mu ? (U(t, g) & 15) : mu;
+            if (mu + mc >= 16) return false;
+        }
         return true;
     }
+};
+
+// The check, evaluated once per stage shape.
+template <class PL, int S, bool MIRROR, bool OUT>
+inline constexpr bool kIndexSplits = IndexSplit<PL, S, MIRROR, OUT>::ok();
+
+// LDS instructions take an unsigned immediate offset below 64 KiB, so a plane larger than
+// that is addressed through one base pointer per 16384-word window: the window is chosen
+// at compile time from the offset, and every access is base + immediate.
+constexpr int kLdsWindow = 16384;
+
+template <int NW>
+struct LdsWindows {
+    float* p[NW];
+    // plane + u is the thread's base; each window's offset is made opaque so that the
+    // compiler keeps one register per window instead of re-adding large constants
+    __device__ __forceinline__ LdsWindows(float* plane, int u)
+    {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            int off = u + w * kLdsWindow;
+            if (w > 0) asm volatile("" : "+v"(off));
+            p[w] = plane + off;
+        }
+    }
+    __device__ __forceinline__ float& at(int c) const { return p[c / kLdsWindow][c % kLdsWindow]; }
 };
 
 // COMP: 1 = real plane, 2 = imaginary plane, 3 = both.
@@ -207,18 +248,21 @@ __device__ __forceinline__ void lds_write(const float2* v, int t, float* re, flo
 {
     using SP = IndexSplit<PL, S, MIRROR, true>;
     constexpr int R = SP::R, B = SP::B;
-    if constexpr (SP::ok()) {
-        const int base0 = pad_any(SP::U(t, 0));
-        const int base1 = MIRROR ? pad_any(SP::U(t, 1)) : 0;
-        const int base2 = MIRROR ? pad_any(SP::U(t, 2)) : 0;
+    if constexpr (kIndexSplits<PL, S, MIRROR, true>) {
+        constexpr int NW = lds_span(PL::N) / kLdsWindow + 1;
+        const int ub[3] = {pad_any(SP::U(t, 0)), MIRROR ? pad_any(SP::U(t, 1)) : 0, MIRROR ? pad_any(SP::U(t, 2)) : 0};
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int base = SP::group(b) == 0 ? base0 : (SP::group(b) == 1 ? base1 : base2);
+        for (int g = 0; g < (MIRROR ? 3 : 1); ++g) {
+            const LdsWindows<NW> wr(re, ub[g]), wi(im, ub[g]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = base + pad_any(SP::C(b, r));
-                if constexpr (COMP & 1) re[i] = v[b * R + r].x;
-                if constexpr (COMP & 2) im[i] = v[b * R + r].y;
+            for (int b = 0; b < B; ++b) {
+                if (SP::group(b) != g) continue;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int c = pad_any(SP::C(b, r));
+                    if constexpr (COMP & 1) wr.at(c) = v[b * R + r].x;
+                    if constexpr (COMP & 2) wi.at(c) = v[b * R + r].y;
+                }
             }
         }
     } else {   // small transforms (fewer than 16 threads) where the split carries
@@ -238,18 +282,21 @@ __device__ __forceinline__ void lds_read(float2* v, int t, const float* re, cons
 {
     using SP = IndexSplit<PL, S, MIRROR, false>;
     constexpr int R = SP::R, B = SP::B;
-    if constexpr (SP::ok()) {
-        const int base0 = pad_any(SP::U(t, 0));
-        const int base1 = MIRROR ? pad_any(SP::U(t, 1)) : 0;
-        const int base2 = MIRROR ? pad_any(SP::U(t, 2)) : 0;
+    if constexpr (kIndexSplits<PL, S, MIRROR, false>) {
+        constexpr int NW = lds_span(PL::N) / kLdsWindow + 1;
+        const int ub[3] = {pad_any(SP::U(t, 0)), MIRROR ? pad_any(SP::U(t, 1)) : 0, MIRROR ? pad_any(SP::U(t, 2)) : 0};
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int base = SP::group(b) == 0 ? base0 : (SP::group(b) == 1 ? base1 : base2);
+        for (int g = 0; g < (MIRROR ? 3 : 1); ++g) {
+            const LdsWindows<NW> wr(const_cast<float*>(re), ub[g]), wi(const_cast<float*>(im), ub[g]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = base + pad_any(SP::C(b, r));
-                if constexpr (COMP & 1) v[b * R + r].x = re[i];
-                if constexpr (COMP & 2) v[b * R + r].y = im[i];
+            for (int b = 0; b < B; ++b) {
+                if (SP::group(b) != g) continue;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int c = pad_any(SP::C(b, r));
+                    if constexpr (COMP & 1) v[b * R + r].x = wr.at(c);
+                    if constexpr (COMP & 2) v[b * R + r].y = wi.at(c);
+                }
             }
         }
     } else {   // small transforms (fewer than 16 threads) where the split carries

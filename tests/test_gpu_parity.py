@@ -202,6 +202,67 @@ def test_host_pointers_are_staged(torch, ctx, kind, k):
     assert np.array_equal(y.view(np.uint32), want.view(np.uint32))
 
 
+def host_call(ctx, kind, n, x, y):
+    if kind == "cfwd":
+        return ctx.complex_forward(n, x, y)
+    if kind == "cinv":
+        return ctx.complex_inverse(n, x, y)
+    if kind == "rfwd":
+        return ctx.real_forward(n, x, y)
+    return ctx.real_inverse(n, x, y, np.zeros(2 * (n // 2 + 1), np.float32))
+
+
+@pytest.mark.parametrize("kind,k", [("cfwd", 17), ("rinv", 20), ("cinv", 21)])
+def test_host_pointers_large_chunked(torch, ctx_big, kind, k):
+    """Host buffers of several MiB go through the chunked pinned pipeline both ways."""
+    n = 1 << k
+    x = make_input(kind, n, 1, 6100 + k)
+    want = run_batch(torch, ctx_big, kind, n, x)[0]
+    y = np.full(out_len(kind, n), np.nan, np.float32)
+    assert host_call(ctx_big, kind, n, np.ascontiguousarray(x[0]), y) == 1
+    assert np.array_equal(y.view(np.uint32), want.view(np.uint32))
+
+
+def test_pinned_and_mixed_buffers(torch, ctx):
+    """Pinned host input (DMA'd directly), device output, and the other way round."""
+    n = 1 << 12
+    x = make_input("cfwd", n, 1, 6200)
+    want = run_batch(torch, ctx, "cfwd", n, x)[0]
+    xp = torch.from_numpy(np.ascontiguousarray(x[0])).pin_memory()
+    yd = torch.full((2 * n,), float("nan"), device="cuda")
+    assert ctx.complex_forward(n, xp, yd) == 1
+    assert np.array_equal(yd.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    xd = torch.from_numpy(np.ascontiguousarray(x[0])).cuda()
+    yp = torch.full((2 * n,), float("nan")).pin_memory()
+    assert ctx.complex_forward(n, xd, yp) == 1
+    assert np.array_equal(yp.numpy().view(np.uint32), want.view(np.uint32))
+
+
+def test_concurrent_host_pointer_calls(torch, ctx):
+    """Eight threads sharing one context (allowed by the reference's header) each run
+    host-pointer calls; every result equals the device-path bits."""
+    import threading
+    cases = [("cfwd", 1 << 14), ("rfwd", 1 << 16), ("rinv", 1 << 13), ("cinv", 1 << 10)]
+    inputs = {c: make_input(c[0], c[1], 1, 6300 + i) for i, c in enumerate(cases)}
+    wants = {c: run_batch(torch, ctx, c[0], c[1], inputs[c])[0] for c in cases}
+    errors = []
+
+    def worker(w):
+        for rep in range(6):
+            kind, n = cases[(w + rep) % len(cases)]
+            y = np.full(out_len(kind, n), np.nan, np.float32)
+            ok = host_call(ctx, kind, n, np.ascontiguousarray(inputs[(kind, n)][0]), y)
+            if ok != 1 or not np.array_equal(y.view(np.uint32), wants[(kind, n)].view(np.uint32)):
+                errors.append((w, rep, kind, n, ok))
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors[:5]
+
+
 def test_max_count_does_not_change_results(torch, C):
     """Like the reference (float twiddle angle scaling by 2^j is exact), results are
     identical for any nMax >= n."""
