@@ -25,7 +25,8 @@ __all__ = [
     "CkFftServerStats", "CkFftServerCreate", "CkFftServerSubmit", "CkFftServerReplay", "CkFftServerQuery",
     "CkFftServerWait", "CkFftServerDrain", "CkFftServerGetStats", "CkFftServerResetStats",
     "CkFftServerDestroy", "CkFftServerLastError", "Server", "CkFftSetPrecision", "CkFftGetPrecision",
-    "kCkFftPrecision_Fp32", "kCkFftPrecision_SplitBf16x3", "EXPORTED_SYMBOLS",
+    "kCkFftPrecision_Fp32", "kCkFftPrecision_SplitBf16x3", "CkFftDeviceTableBytes", "CkFftInitArena",
+    "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
@@ -122,6 +123,8 @@ _SIGS = {
     "CkFftServerDestroy": (None, [_vp]),
     "CkFftServerLastError": (ctypes.c_char_p, []),
     "CkFftSetPrecision": (_i, [_vp, _i]),
+    "CkFftDeviceTableBytes": (ctypes.c_size_t, [_i, _i]),
+    "CkFftInitArena": (_vp, [_i, _i, _vp, ctypes.POINTER(ctypes.c_size_t), _vp, ctypes.c_size_t]),
     "CkFftGetPrecision": (_i, [_vp]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -158,10 +161,17 @@ class Context:
     Buffers may be torch tensors (device or host), numpy arrays or raw addresses.
     """
 
-    def __init__(self, n_max: int, direction: int = kCkFftDirection_Both):
+    def __init__(self, n_max: int, direction: int = kCkFftDirection_Both, arena=None):
+        """``arena``: optional device buffer (at least CkFftDeviceTableBytes bytes) that
+        holds the context's tables instead of memory the library allocates."""
         self.n_max = n_max
         self.direction = direction
-        self.handle = CkFftInit(n_max, direction, None, None)
+        self.arena = arena                       # keeps a caller's arena alive
+        if arena is None:
+            self.handle = CkFftInit(n_max, direction, None, None)
+        else:
+            nbytes = arena.numel() * arena.element_size() if hasattr(arena, "numel") else None
+            self.handle = CkFftInitArena(n_max, direction, None, None, _ptr(arena), nbytes or 0)
         if not self.handle:
             raise RuntimeError(f"CkFftInit({n_max}, {direction}) failed: "
                                f"{CkFftGpuLastError().decode()} (or invalid size/direction)")

@@ -120,17 +120,41 @@ int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void
     return record(e, "launch") ? 1 : 0;
 }
 
-}  // namespace
+// The context's device tables, in one block: the pyramid, the fine tables (levels above
+// 2^16) and the transposed inter-pass table, each 256-byte aligned.
+struct TableLayout {
+    int log_pyr, log_max, log_xtw;
+    size_t pyr_off, fine_off, xtw_off, total;
+};
 
-extern "C" {
-
-// src/ckfft/ckfft.cpp:14-34 and src/ckfft/context.cpp:45-135
-CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* bufSize)
+TableLayout table_layout(int nMax)
 {
-    if (!pow2_positive(nMax)) return nullptr;
-    if (direction != kCkFftDirection_Forward && direction != kCkFftDirection_Inverse &&
-        direction != kCkFftDirection_Both)
-        return nullptr;
+    TableLayout t;
+    t.log_pyr = std::max(1, std::min(ilog2(nMax), kLogMaxPyramid));
+    t.log_max = std::min(ilog2(nMax), kLogMaxSupported);
+    t.log_xtw = t.log_pyr >= kLogMinXtw ? t.log_pyr : 0;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    t.pyr_off = 0;
+    t.fine_off = al(sizeof(float2) * ((size_t)2 << t.log_pyr));
+    const size_t fine_bytes =
+        t.log_max > kLogMaxPyramid ? sizeof(float2) * ((size_t)(t.log_max - kLogMaxPyramid) << kLogFine) : 0;
+    t.xtw_off = t.fine_off + al(fine_bytes);
+    t.total = t.xtw_off + al(t.log_xtw ? sizeof(float2) << t.log_xtw : 0);
+    return t;
+}
+
+bool valid_init_args(int nMax, CkFftDirection direction)
+{
+    return pow2_positive(nMax) && (direction == kCkFftDirection_Forward || direction == kCkFftDirection_Inverse ||
+                                   direction == kCkFftDirection_Both);
+}
+
+// CkFftInit with the device tables in `arena` (caller's device memory) or, when arena is
+// null, in one block the library allocates.
+CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t* bufSize, void* arena,
+                           size_t arena_bytes)
+{
+    if (!valid_init_args(nMax, direction)) return nullptr;
     if (buf && !bufSize) return nullptr;
     const size_t need = host_context_bytes();
     if (bufSize && (!buf || *bufSize < need)) {
@@ -143,25 +167,24 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
         (void)hipGetLastError();
         return nullptr;
     }
-    const int log_pyr = std::max(1, std::min(ilog2(nMax), kLogMaxPyramid));
-    float2* pyr = nullptr;
-    if (!record(hipMalloc(&pyr, sizeof(float2) * ((size_t)2 << log_pyr)), "hipMalloc(twiddles)")) return nullptr;
-    const int log_max = std::min(ilog2(nMax), kLogMaxSupported);
-    float2* fine = nullptr;
-    float2* xtw = nullptr;
-    const int log_xtw = log_pyr >= kLogMinXtw ? log_pyr : 0;
-    auto release = [&]() {
-        (void)hipFree(pyr);
-        if (fine) (void)hipFree(fine);
-        if (xtw) (void)hipFree(xtw);
-    };
-    if ((log_max > kLogMaxPyramid &&
-         !record(hipMalloc(&fine, sizeof(float2) * ((size_t)(log_max - kLogMaxPyramid) << kLogFine)),
-                 "hipMalloc(fine twiddles)")) ||
-        (log_xtw && !record(hipMalloc(&xtw, sizeof(float2) << log_xtw), "hipMalloc(pass twiddles)"))) {
-        release();
+    const TableLayout L = table_layout(nMax);
+    char* block = (char*)arena;
+    if (arena) {
+        if (arena_bytes < L.total || ((uintptr_t)arena & 255) || !is_device_pointer(arena)) {
+            std::snprintf(g_last_error, sizeof(g_last_error), "device arena: need %zu bytes of 256-aligned device memory",
+                          L.total);
+            return nullptr;
+        }
+    } else if (!record(hipMalloc((void**)&block, L.total), "hipMalloc(twiddles)")) {
         return nullptr;
     }
+    float2* pyr = (float2*)(block + L.pyr_off);
+    float2* fine = L.log_max > kLogMaxPyramid ? (float2*)(block + L.fine_off) : nullptr;
+    float2* xtw = L.log_xtw ? (float2*)(block + L.xtw_off) : nullptr;
+    const int log_pyr = L.log_pyr, log_max = L.log_max, log_xtw = L.log_xtw;
+    auto release = [&]() {
+        if (!arena) (void)hipFree(block);
+    };
     if (!record(launch_twiddle_pyramid(pyr, log_pyr, nullptr), "twiddle launch") ||
         !record(launch_twiddle_fine(fine, log_max, nullptr), "fine twiddle launch") ||
         !record(launch_twiddle_transposed(xtw, pyr, log_xtw, nullptr), "pass twiddle launch") ||
@@ -192,11 +215,35 @@ CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* b
     c->d_fine = fine;
     c->d_xtw = xtw;
     c->log_xtw = log_xtw;
+    c->d_tables = block;
+    c->own_device = arena ? 0 : 1;
     c->log_max = log_max;
     std::memset(c->tune, 0, sizeof(c->tune));
     c->precision = kCkFftPrecision_Fp32;
     c->staging = pool;
     return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+// src/ckfft/ckfft.cpp:14-34 and src/ckfft/context.cpp:45-135
+CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* bufSize)
+{
+    return init_context(nMax, direction, buf, bufSize, nullptr, 0);
+}
+
+size_t CkFftDeviceTableBytes(int nMax, CkFftDirection direction)
+{
+    return valid_init_args(nMax, direction) ? table_layout(nMax).total : 0;
+}
+
+CkFftContext* CkFftInitArena(int nMax, CkFftDirection direction, void* buf, size_t* bufSize, void* deviceArena,
+                             size_t deviceArenaBytes)
+{
+    if (!deviceArena && !(bufSize && !buf)) return nullptr;   // an arena is required (query mode aside)
+    return init_context(nMax, direction, buf, bufSize, deviceArena, deviceArenaBytes);
 }
 
 // src/ckfft/ckfft.cpp:36-53
@@ -240,9 +287,7 @@ int CkFftComplexInverse(CkFftContext* context, int n, const CkFftComplex* input,
 void CkFftShutdown(CkFftContext* context)
 {
     if (!context || context->magic != kContextMagic) return;
-    (void)hipFree(context->d_pyramid);
-    if (context->d_fine) (void)hipFree(context->d_fine);
-    if (context->d_xtw) (void)hipFree(context->d_xtw);
+    if (context->own_device) (void)hipFree(context->d_tables);
     staging_destroy(context->staging);
     const int own = context->own_host;
     context->magic = 0;

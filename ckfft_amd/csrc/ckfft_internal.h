@@ -101,6 +101,8 @@ struct _CkFftContext {
     float2* d_fine;          // fine twiddle tables for levels above 2^16 (or null)
     float2* d_xtw;           // transposed inter-pass twiddles (or null)
     int log_xtw;             // their level, 0 when absent
+    void* d_tables;          // the one device block holding the three tables above
+    int own_device;          // 1 if the library allocated d_tables (else a caller's arena)
     int log_max;             // largest log2(n) the tables serve
     // Tuned variant (points per thread 16 or 32) per [kind][log2 complex size], used by
     // batch plans; every variant has the same arithmetic.

@@ -163,6 +163,22 @@ int CkFftSetPrecision(CkFftContext* context, int precision);
 /* The context's mode, or -1 for an invalid context. */
 int CkFftGetPrecision(const CkFftContext* context);
 
+/* ---- device memory of a context ------------------------------------------------------
+ *
+ * CkFftInit's query mode (buf == NULL, bufSize != NULL) reports the host part of a context.
+ * Its device part -- the twiddle tables, generated in FP64 on the GPU at init -- is one
+ * block whose size CkFftDeviceTableBytes reports (0 for invalid arguments).  CkFftInitArena
+ * is CkFftInit with that block placed in caller-provided device memory (`deviceArena`, at
+ * least CkFftDeviceTableBytes bytes, 256-byte aligned); the arena stays the caller's and
+ * CkFftShutdown does not free it (it may be reused once the context is shut down).  Query
+ * mode works as in CkFftInit.  Host-pointer calls, batch plans and servers still allocate
+ * their own buffers.  NULL on invalid arguments or a too small / misaligned / non-device
+ * arena (CkFftGpuLastError tells which).  The reference has no counterpart (its context
+ * holds host tables only: src/ckfft/context.cpp:26-67). */
+size_t CkFftDeviceTableBytes(int nMax, CkFftDirection direction);
+CkFftContext* CkFftInitArena(int nMax, CkFftDirection direction, void* buf, size_t* bufSize, void* deviceArena,
+                             size_t deviceArenaBytes);
+
 /* Copy the device twiddle table of level L (L values exp(-2 pi i k / L), k < L,
  * generated in FP64 on the GPU and rounded to FP32) to host memory `out`.
  * L must be a power of two no larger than the context's table.  Returns 1/0. */

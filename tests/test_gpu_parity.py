@@ -263,6 +263,34 @@ def test_concurrent_host_pointer_calls(torch, ctx):
     assert not errors, errors[:5]
 
 
+def test_device_arena_context(torch, C):
+    """Tables in a caller's device arena: same bits as a library-allocated context; the
+    arena is left to the caller at shutdown and can hold a new context; bad arenas refused."""
+    assert C.CkFftDeviceTableBytes(1024, C.kCkFftDirection_Both) == 2 * 1024 * 8
+    assert C.CkFftDeviceTableBytes(1 << 16, C.kCkFftDirection_Both) == (2 << 16) * 8 + (1 << 16) * 8
+    assert C.CkFftDeviceTableBytes(1 << 20, C.kCkFftDirection_Both) == (2 << 16) * 8 + 4 * 4096 * 8 + (1 << 16) * 8
+    assert C.CkFftDeviceTableBytes(1000, C.kCkFftDirection_Both) == 0
+    assert C.CkFftDeviceTableBytes(1024, 7) == 0
+    n = 1 << 16
+    need = C.CkFftDeviceTableBytes(n, C.kCkFftDirection_Both)
+    arena = torch.zeros(need + 512, dtype=torch.uint8, device="cuda")
+    x = make_input("cfwd", n, 2, 6400)
+    r = make_input("rfwd", n, 2, 6401)
+    with C.Context(n) as plain_ctx:
+        want_c = run_batch(torch, plain_ctx, "cfwd", n, x)
+        want_r = run_batch(torch, plain_ctx, "rfwd", n, r)
+    for rep in range(2):                                # the arena is reused after shutdown
+        with C.Context(n, C.kCkFftDirection_Both, arena=arena[:need]) as ac:
+            assert np.array_equal(run_batch(torch, ac, "cfwd", n, x).view(np.uint32), want_c.view(np.uint32))
+            assert np.array_equal(run_batch(torch, ac, "rfwd", n, r).view(np.uint32), want_r.view(np.uint32))
+    for bad in (arena[: need - 256], arena[1:need + 1], torch.zeros(need, dtype=torch.uint8)):
+        with pytest.raises(RuntimeError):
+            C.Context(n, C.kCkFftDirection_Both, arena=bad)
+    size = ctypes.c_size_t(0)
+    assert not C.CkFftInitArena(n, C.kCkFftDirection_Both, None, ctypes.byref(size), None, 0)
+    assert size.value > 0                               # query mode reports the host part
+
+
 def test_max_count_does_not_change_results(torch, C):
     """Like the reference (float twiddle angle scaling by 2^j is exact), results are
     identical for any nMax >= n."""
