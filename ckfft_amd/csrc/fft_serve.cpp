@@ -43,16 +43,20 @@ uint64_t now_ns()
 
 enum ReqState : int { kQueued = 0, kAdmitted = 1, kLaunched = 2, kDone = 3, kFailed = 4 };
 
+// One entry of the request ring (all fields guarded by the server mutex).
 struct Request {
-    CkFftBatchDesc d;
-    long long bytes;        // algorithmic bytes, charged against the budget
-    uint64_t t_arrive;      // arrival (submission or scheduled arrival) time
-    uint64_t t_done;
-    std::atomic<int> state;
-    Request(const CkFftBatchDesc& dd, long long b, uint64_t t) : d(dd), bytes(b), t_arrive(t), t_done(0), state(kQueued)
-    {
-    }
+    CkFftBatchDesc d{};
+    long long bytes = 0;     // algorithmic bytes, charged against the budget
+    uint64_t t_arrive = 0;   // arrival (submission or scheduled arrival) time
+    long long ticket = -1;   // the submission this entry holds
+    int state = kDone;
 };
+
+// Requests live in a ring indexed by ticket; an entry is reused once its request has
+// finished (a submitter waits while the entry it needs is still pending).  Latencies are
+// kept for the most recent completions only.
+constexpr long long kRing = 1 << 18;
+constexpr size_t kLatencyWindow = 1 << 20;
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -81,7 +85,8 @@ struct _CkFftServer {
     long long resident = 0;              // bytes the server itself holds on the device
     int grid = 0;
     std::vector<Slot> slots;
-    std::deque<Request> reqs;            // ticket = index; deque keeps addresses stable
+    std::vector<Request> ring = std::vector<Request>(kRing);
+    long long next_ticket = 0;
     std::mutex m;
     std::condition_variable cv;
     std::deque<long long> queued;        // submitted, not admitted
@@ -91,8 +96,9 @@ struct _CkFftServer {
     long long outstanding = 0;
     std::atomic<bool> stop{false};
     std::thread worker;
-    // statistics (guarded by m)
+    // statistics (guarded by m): latencies of the last kLatencyWindow completions
     std::vector<double> lat_us;
+    size_t lat_next = 0;
     long long completed = 0, failed = 0, batches = 0, batched = 0, done_bytes = 0;
     uint64_t t_first = 0, t_last = 0;
 };
@@ -114,16 +120,20 @@ void finish_slot(CkFftServer* S, Slot& sl, bool ok)
     {
         std::lock_guard<std::mutex> lock(S->m);
         for (long long id : sl.reqs) {
-            Request& r = S->reqs[id];
-            r.t_done = t;
+            Request& r = S->ring[id % kRing];
             if (ok) {
-                S->lat_us.push_back((double)(t - r.t_arrive) * 1e-3);
+                const double us = (double)(t - r.t_arrive) * 1e-3;
+                if (S->lat_us.size() < kLatencyWindow)
+                    S->lat_us.push_back(us);
+                else
+                    S->lat_us[S->lat_next] = us;
+                S->lat_next = (S->lat_next + 1) % kLatencyWindow;
                 ++S->completed;
                 S->done_bytes += r.bytes;
             } else {
                 ++S->failed;
             }
-            r.state.store(ok ? kDone : kFailed);
+            r.state = ok ? kDone : kFailed;
         }
         S->inflight_bytes -= sl.bytes;
         S->outstanding -= (long long)sl.reqs.size();
@@ -170,7 +180,7 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
     sl.busy = true;
     {
         std::lock_guard<std::mutex> lock(S->m);
-        for (long long id : sl.reqs) S->reqs[id].state.store(kLaunched);
+        for (long long id : sl.reqs) S->ring[id % kRing].state = kLaunched;
         ++S->batches;
         S->batched += (long long)sl.reqs.size();
     }
@@ -210,11 +220,11 @@ void dispatcher(CkFftServer* S)
         {
             std::unique_lock<std::mutex> lock(S->m);
             while (!S->queued.empty()) {
-                Request& r = S->reqs[S->queued.front()];
+                Request& r = S->ring[S->queued.front() % kRing];
                 if (S->inflight_bytes + r.bytes > S->budget) break;      // back-pressure
                 S->inflight_bytes += r.bytes;
                 S->peak_inflight = std::max(S->peak_inflight, S->inflight_bytes);
-                r.state.store(kAdmitted);
+                r.state = kAdmitted;
                 S->admitted.push_back(S->queued.front());
                 S->queued.pop_front();
             }
@@ -223,7 +233,7 @@ void dispatcher(CkFftServer* S)
                 size_t take = 0;
                 int two = 0;
                 while (take < S->admitted.size() && (int)take < kMaxBatch) {
-                    const CkFftBatchDesc& d = S->reqs[S->admitted[take]].d;
+                    const CkFftBatchDesc& d = S->ring[S->admitted[take] % kRing].d;
                     if (d.kind <= kComplexInverse && d.n == (1 << kPassLogN)) {
                         if (two == kScratchPerSlot) break;
                         ++two;
@@ -235,8 +245,8 @@ void dispatcher(CkFftServer* S)
                 descs.clear();
                 bytes = 0;
                 for (long long id : batch) {
-                    descs.push_back(S->reqs[id].d);
-                    bytes += S->reqs[id].bytes;
+                    descs.push_back(S->ring[id % kRing].d);
+                    bytes += S->ring[id % kRing].bytes;
                 }
             }
             if (batch.empty() && !progress && S->outstanding == 0) {
@@ -332,9 +342,16 @@ static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_a
     }
     long long id;
     {
-        std::lock_guard<std::mutex> lock(S->m);
-        id = (long long)S->reqs.size();
-        S->reqs.emplace_back(d, bytes, t_arrive);
+        std::unique_lock<std::mutex> lock(S->m);
+        id = S->next_ticket++;
+        Request& r = S->ring[id % kRing];
+        // the entry's previous request (kRing submissions ago) must have finished
+        S->cv.wait(lock, [&] { return r.state >= kDone; });
+        r.d = d;
+        r.bytes = bytes;
+        r.t_arrive = t_arrive;
+        r.ticket = id;
+        r.state = kQueued;
         S->queued.push_back(id);
         ++S->outstanding;
         if (!S->t_first) S->t_first = t_arrive;
@@ -370,19 +387,20 @@ int CkFftServerQuery(CkFftServer* server, long long ticket)
 {
     if (!server || ticket < 0) return -1;
     std::lock_guard<std::mutex> lock(server->m);
-    if (ticket >= (long long)server->reqs.size()) return -1;
-    const int st = server->reqs[ticket].state.load();
-    return st == kDone ? 1 : (st == kFailed ? -1 : 0);
+    if (ticket >= server->next_ticket) return -1;
+    const Request& r = server->ring[ticket % kRing];
+    if (r.ticket != ticket) return 1;          // older than the ring: finished long ago
+    return r.state == kDone ? 1 : (r.state == kFailed ? -1 : 0);
 }
 
 int CkFftServerWait(CkFftServer* server, long long ticket)
 {
     if (!server || ticket < 0) return 0;
     std::unique_lock<std::mutex> lock(server->m);
-    if (ticket >= (long long)server->reqs.size()) return 0;
-    Request& r = server->reqs[ticket];
-    server->cv.wait(lock, [&] { return r.state.load() >= kDone; });
-    return r.state.load() == kDone ? 1 : 0;
+    if (ticket >= server->next_ticket) return 0;
+    const Request& r = server->ring[ticket % kRing];
+    server->cv.wait(lock, [&] { return r.ticket != ticket || r.state >= kDone; });
+    return r.ticket != ticket || r.state == kDone ? 1 : 0;
 }
 
 int CkFftServerDrain(CkFftServer* server)
@@ -425,6 +443,7 @@ void CkFftServerResetStats(CkFftServer* server)
     if (!server) return;
     std::lock_guard<std::mutex> lock(server->m);
     server->lat_us.clear();
+    server->lat_next = 0;
     server->completed = server->failed = server->batches = server->batched = server->done_bytes = 0;
     server->peak_inflight = server->inflight_bytes;
     server->t_first = server->t_last = 0;

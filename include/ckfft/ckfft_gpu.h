@@ -111,7 +111,8 @@ typedef struct _CkFftServer CkFftServer;
 typedef struct {
     long long completed;          /* requests finished since the last reset */
     long long failed;
-    double p50_us, p99_us, max_us, mean_us;  /* latency, arrival to completion seen */
+    double p50_us, p99_us, max_us, mean_us;  /* latency, arrival to completion seen, over the
+                                              last 2^20 completions since the last reset */
     double requests_per_s;        /* completed / (last completion - first arrival) */
     double GB_per_s;              /* algorithmic bytes over the same span */
     long long peak_inflight_bytes;/* largest admitted-unfinished byte count */
@@ -125,14 +126,17 @@ typedef struct {
  * allocation failure (CkFftServerLastError tells which).  maxInFlight in [1, 64]. */
 CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxInFlight);
 /* Submit one transform (kind, n and buffers as in CkFftBatchDesc).  Returns a ticket
- * >= 0, or -1 if the request is invalid or alone exceeds the budget. */
+ * >= 0, or -1 if the request is invalid or alone exceeds the budget.  The server tracks
+ * the last 2^18 tickets; a submission waits while the ticket 2^18 before it is still
+ * unfinished. */
 long long CkFftServerSubmit(CkFftServer* server, int kind, int n, const void* input, void* output);
 /* Submit descs[i] at arrivalUs[i] microseconds after the call (non-decreasing times),
  * from the calling thread, writing tickets[i] (may be NULL).  Latency is measured from
  * the scheduled arrival.  1 if every request was accepted. */
 int CkFftServerReplay(CkFftServer* server, const CkFftBatchDesc* descs, const double* arrivalUs, int count,
                       long long* tickets);
-/* 1 done, 0 pending, -1 failed or unknown ticket. */
+/* 1 done, 0 pending, -1 failed or unknown ticket.  A ticket more than 2^18 submissions
+ * old has finished and reports 1 (its failure, if any, is counted in the stats). */
 int CkFftServerQuery(CkFftServer* server, long long ticket);
 /* Block until the ticket completes; 1 on success. */
 int CkFftServerWait(CkFftServer* server, long long ticket);

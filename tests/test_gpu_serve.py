@@ -160,3 +160,23 @@ def test_concurrent_submitters(torch, C, ctx):
         assert srv.drain() == 1
         assert srv.stats()["completed"] == len(mix)
     assert_alone_equals_mix(torch, ctx, mix, ins, outs)
+
+
+def test_ticket_ring_wraps(torch, C, ctx):
+    """More submissions than the server's ticket ring (2^18): old entries are reused once
+    finished, every request completes, old tickets report done."""
+    n = 16
+    x = torch.from_numpy(np.ascontiguousarray(O.uniform_pm1(56, 2 * n))).cuda()
+    y = torch.full((2 * n,), float("nan"), device="cuda")
+    want = plain(torch, ctx, "cfwd", n, x).cpu().numpy()
+    count = (1 << 18) + 5000
+    torch.cuda.synchronize()
+    with C.Server(ctx, 1 << 30, max_in_flight=4) as srv:
+        tickets = srv.replay([(0, n, x, y)] * count, np.zeros(count))   # same buffers: identical writes
+        assert srv.drain() == 1
+        st = srv.stats()
+        assert st["completed"] == count and st["failed"] == 0
+        assert tickets[-1] == count - 1
+        assert srv.query(0) == 1 and srv.query(count - 1) == 1 and srv.wait(1) == 1
+        assert srv.query(count) == -1                   # not issued yet
+    assert np.array_equal(y.cpu().numpy().view(np.uint32), want.view(np.uint32))
