@@ -81,3 +81,22 @@ def test_product_does_not_reference_the_oracle():
         if os.path.isfile(path) and path.endswith((".py", ".cpp", ".hip", ".h")):
             text = open(path).read()
             assert "import oracle" not in text and "libckfft_oracle" not in text and "_ref/" not in text, path
+
+
+def test_headers_compile_as_c_and_link(tmp_path):
+    """The headers compile as strict C (the reference's own test includes its header from
+    C, src/test/test.c:1-3), and a C program taking the address of every declared function
+    links against the library."""
+    names = sorted(declared_functions())
+    src = tmp_path / "use_api.c"
+    src.write_text(
+        '#include "ckfft/ckfft.h"\n#include "ckfft/ckfft_gpu.h"\n#include <stdio.h>\n'
+        "int main(void) {\n"
+        + "".join(f'    printf("%p\\n", (void*)&{n});\n' for n in names)
+        + "    return 0;\n}\n")
+    exe = tmp_path / "use_api"
+    cmd = ["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-Wno-pedantic",
+           "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           "-L", os.path.dirname(LIB), "-lckfft_amd", "-Wl,--unresolved-symbols=ignore-in-shared-libs"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
