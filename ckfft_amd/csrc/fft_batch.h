@@ -57,8 +57,11 @@ struct BatchBuild {
 // false records the reason for CkFftBatchLastError.
 bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int index, bool check_pointers);
 // Validate and group a batch; var_override >= 0 forces one thread layout for every class.
+// twopass_as_extra: complex 2^16 transforms run as ordinary multi-pass launches (many
+// workgroups each, the lower latency for a lone transform) instead of persistent items
+// (one workgroup each); both compute the same two radix-2^8 passes.
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out);
+                 bool check_pointers, BatchBuild& out, bool twopass_as_extra = false);
 void batch_assign_scratch(BatchBuild& b, float2* scratch);
 int persistent_grid(const CkFftContext* ctx);
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,

@@ -271,7 +271,7 @@ bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int inde
 }
 
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out)
+                 bool check_pointers, BatchBuild& out, bool twopass_as_extra)
 {
     out = BatchBuild{};
     struct Entry {
@@ -289,6 +289,8 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         // the persistent kernel computes in FP32; transforms of the split-BF16x3 mode run
         // as ordinary launches of that mode's kernels
         if (ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n)) ok = false;
+        // the same two radix-2^8 passes, as separate launches over many workgroups
+        if (twopass_as_extra && !real && log2n == kPassLogN) ok = false;
         if (ok)
             pers.push_back({d.kind, log2n, logc, i});
         else
@@ -514,6 +516,7 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath)
         !batch_error(hipMalloc(&out, kBytesPerClass + 4096 * 8), "hipMalloc(tune out)") ||
         !batch_error(hipMemset(in, 0, kBytesPerClass), "clear tune input")) {
         if (in) (void)hipFree(in);
+        if (out) (void)hipFree(out);
         return 0;
     }
     hipEvent_t e0, e1;

@@ -64,14 +64,12 @@ struct Slot {
     char* h_desc = nullptr;              // pinned staging of the batch's queue and transform table
     char* d_desc = nullptr;              // their device copy
     int* d_ctl = nullptr;
-    float2* d_scratch = nullptr;
     std::vector<long long> reqs;
     long long bytes = 0;
     bool busy = false;
 };
 
 constexpr int kMaxBatch = 2048;          // transforms per batch (slot buffer capacity)
-constexpr int kScratchPerSlot = 32;      // complex 2^16 transforms per batch
 constexpr uint64_t kSpinNs = 2000000;    // idle dispatcher spins this long before sleeping
 // queue entries first, the transform table after them at a 256-byte boundary, so one
 // copy uploads both
@@ -152,8 +150,9 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
 {
     BatchBuild b;
     // descriptors were validated at submission
-    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b)) return false;
-    batch_assign_scratch(b, sl.d_scratch);
+    // requests arrive a few at a time: a complex 2^16 one is faster as its own multi-pass
+    // launch than as one workgroup's item
+    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b, true)) return false;
     const DeviceTables tb = context_tables(S->ctx);
     bool ok = true;
     if (!b.items.empty() && b.items.size() <= (size_t)kInlineQueue && b.trs.size() <= (size_t)kInlineQueue) {
@@ -229,17 +228,8 @@ void dispatcher(CkFftServer* S)
                 S->queued.pop_front();
             }
             if (free_slot && !S->admitted.empty()) {
-                // take admitted requests in order, within the slot's capacities
-                size_t take = 0;
-                int two = 0;
-                while (take < S->admitted.size() && (int)take < kMaxBatch) {
-                    const CkFftBatchDesc& d = S->ring[S->admitted[take] % kRing].d;
-                    if (d.kind <= kComplexInverse && d.n == (1 << kPassLogN)) {
-                        if (two == kScratchPerSlot) break;
-                        ++two;
-                    }
-                    ++take;
-                }
+                // take admitted requests in order, up to the slot's queue capacity
+                const size_t take = std::min(S->admitted.size(), (size_t)kMaxBatch);
                 batch.assign(S->admitted.begin(), S->admitted.begin() + take);
                 S->admitted.erase(S->admitted.begin(), S->admitted.begin() + take);
                 descs.clear();
@@ -282,7 +272,6 @@ void destroy_slots(CkFftServer* S)
         if (sl.h_desc) (void)hipHostFree(sl.h_desc);
         if (sl.d_desc) (void)hipFree(sl.d_desc);
         if (sl.d_ctl) (void)hipFree(sl.d_ctl);
-        if (sl.d_scratch) (void)hipFree(sl.d_scratch);
     }
     S->slots.clear();
 }
@@ -298,21 +287,18 @@ CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxI
     if (!S) return nullptr;
     S->ctx = context;
     S->slots.resize(maxInFlight);
-    const bool two_pass = context->log_max >= kPassLogN;
-    const size_t scratch_bytes = two_pass ? sizeof(float2) * ((size_t)kScratchPerSlot << kPassLogN) : 0;
     for (Slot& sl : S->slots) {
         if (!serr(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "stream") ||
             !serr(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "event") ||
             !serr(hipHostMalloc((void**)&sl.h_desc, kDescBytes, hipHostMallocDefault), "pinned queue") ||
             !serr(hipMalloc((void**)&sl.d_desc, kDescBytes), "queue") ||
             !serr(hipMalloc(&sl.d_ctl, sizeof(int) * kCtlWords), "control") ||
-            !serr(hipMemset(sl.d_ctl, 0, sizeof(int) * kCtlWords), "clear control") ||
-            (scratch_bytes && !serr(hipMalloc(&sl.d_scratch, scratch_bytes), "scratch"))) {
+            !serr(hipMemset(sl.d_ctl, 0, sizeof(int) * kCtlWords), "clear control")) {
             destroy_slots(S);
             delete S;
             return nullptr;
         }
-        S->resident += (long long)(scratch_bytes + kDescBytes + sizeof(int) * kCtlWords);
+        S->resident += (long long)(kDescBytes + sizeof(int) * kCtlWords);
     }
     S->budget = (long long)hbmBudget - S->resident;
     S->grid = persistent_grid(context);
@@ -335,7 +321,11 @@ static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_a
     }
     int log2n = 0;
     while ((1 << log2n) < d.n) ++log2n;
-    const long long bytes = transform_bytes(d.kind, log2n);
+    // algorithmic bytes, plus the intermediate a multi-pass launch allocates for itself
+    const bool real = d.kind >= kRealForward;
+    const bool multipass = real ? log2n > kLogMaxLds + 1 : log2n > kLogMaxLds;
+    const bool mfma = S->ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n);
+    const long long bytes = transform_bytes(d.kind, log2n) + (multipass && !mfma ? 8ll << log2n : 0);
     if (bytes > S->budget) {
         std::snprintf(g_serve_error, sizeof(g_serve_error), "request larger than the admission budget");
         return -1;

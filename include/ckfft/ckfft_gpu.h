@@ -96,12 +96,14 @@ int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
  * A server admits transforms as they arrive, from any thread, and runs them with
  * continuous batching: a dispatcher thread takes every admitted request into the next
  * free slot (up to maxInFlight batches on the device at once, each one persistent
- * launch on the slot's own stream), so requests arriving while others run join the
- * next batch and their work overlaps on the device.  Admission is first-come-first-
- * served against an HBM budget: a request is admitted while the algorithmic bytes of
- * admitted, unfinished requests (one read of the input, one write of the output) plus
- * the server's own resident buffers stay within hbmBudget; others wait (back-pressure,
- * not failure).  Twiddles, descriptor buffers, scratch and streams stay resident.
+ * launch on the slot's own stream; transforms above one workgroup's size -- complex
+ * n >= 2^16, real n >= 2^17 -- run as multi-pass launches on that stream), so requests
+ * arriving while others run join the next batch and their work overlaps on the device.
+ * Admission is first-come-first-served against an HBM budget: a request is admitted
+ * while the bytes of admitted, unfinished requests (one read of the input, one write of
+ * the output, plus 8n bytes of intermediate for the multi-pass sizes) plus the server's
+ * own resident buffers stay within hbmBudget; others wait (back-pressure, not failure).
+ * Twiddles, descriptor buffers and streams stay resident.
  * A request's result is bit-identical to the single-signal call on the same input,
  * whatever it was batched with.  Buffers are device memory owned by the caller and
  * must stay valid until the request completes.  The reference has no counterpart. */
