@@ -74,7 +74,7 @@ class CkFftBatchDesc(ctypes.Structure):
 class CkFftBatchInfo(ctypes.Structure):
     _fields_ = [("transforms", ctypes.c_int), ("persistent_transforms", ctypes.c_int),
                 ("work_items", ctypes.c_int), ("workgroups", ctypes.c_int), ("extra_launches", ctypes.c_int),
-                ("algorithmic_bytes", ctypes.c_longlong)]
+                ("algorithmic_bytes", ctypes.c_longlong), ("pass_transforms", ctypes.c_int)]
 
 
 class CkFftServerStats(ctypes.Structure):

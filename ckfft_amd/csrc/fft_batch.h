@@ -16,11 +16,11 @@ struct BatchItem {
     int pad;
 };
 
-// The buffers of one transform taken by the persistent kernel.
+// The buffers of one transform taken by the persistent kernel or the table passes.
 struct BatchTransform {
     const float2* in;
     float2* out;
-    float2* scratch;   // intermediate of a complex 2^16 transform (else null)
+    float2* scratch;   // intermediate of a table-pass transform (else null)
 };
 
 // A transform the persistent kernel does not take: run with an ordinary launch.
@@ -32,7 +32,7 @@ struct BatchExtra {
 };
 
 constexpr int kPersistWG = 512;       // threads per persistent workgroup
-constexpr int kPassLogN = 16;         // complex size run as two passes inside one workgroup
+constexpr int kPassLogN = 16;         // complex size run as two global passes (not by the persistent kernel)
 constexpr int kCtlWords = 2;          // control words of one launch: queue head, finished count
 
 // A queue small enough to travel in the kernel arguments (no upload before the launch).
@@ -46,9 +46,11 @@ struct InlineQueue {
 // Host description of a batch.
 struct BatchBuild {
     std::vector<BatchItem> items;
-    std::vector<BatchTransform> trs;  // scratch pointers set by batch_assign_scratch
+    std::vector<BatchTransform> trs;  // the persistent kernel's transforms, by class
     std::vector<BatchExtra> extras;
-    std::vector<int> twopass;         // transforms needing (1 << kPassLogN) points of scratch
+    // complex 2^16 transforms run by the table pass launches (TwoPass::kTable), forward
+    // then inverse; scratch set by batch_assign_scratch
+    std::vector<BatchTransform> pass_fwd, pass_inv;
     long long bytes = 0;              // algorithmic bytes of the whole batch
     int persistent = 0;               // transforms taken by the persistent kernel
 };
@@ -56,18 +58,27 @@ struct BatchBuild {
 // Validation of one descriptor (single-signal rules, plus device buffers when asked);
 // false records the reason for CkFftBatchLastError.
 bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int index, bool check_pointers);
+// Where complex 2^16 transforms run.  Both compute the same two radix-2^8 passes:
+//  kExtra  an ordinary multi-pass launch each (the lower latency for a lone transform,
+//          used by the server);
+//  kTable  two launches per direction over all of them, tiles spread over the whole GPU
+//          (the higher throughput for many, used by batch plans).
+enum class TwoPass { kExtra, kTable };
+
 // Validate and group a batch; var_override >= 0 forces one thread layout for every class.
-// twopass_as_extra: complex 2^16 transforms run as ordinary multi-pass launches (many
-// workgroups each, the lower latency for a lone transform) instead of persistent items
-// (one workgroup each); both compute the same two radix-2^8 passes.
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out, bool twopass_as_extra = false);
+                 bool check_pointers, BatchBuild& out, TwoPass two_pass);
+// Scratch of the table-pass transforms: (1 << kPassLogN) points each.
+size_t batch_scratch_transforms(const BatchBuild& b);
 void batch_assign_scratch(BatchBuild& b, float2* scratch);
+// The table pass launches: pass 1 then pass 2 over `count` transforms of d_trs.
+hipError_t launch_pass_table(const BatchTransform* d_trs, int count, bool inverse, const DeviceTables& tb,
+                             hipStream_t s);
 int persistent_grid(const CkFftContext* ctx);
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
                                  int grid, const DeviceTables& tb, hipStream_t s);
-// The same launch with the queue of `b` (scratch assigned) passed by value; requires at
-// most kInlineQueue items and transforms.
+// The same launch with the queue of `b` passed by value; requires at most kInlineQueue
+// items and transforms.
 hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, const DeviceTables& tb, hipStream_t s);
 DeviceTables context_tables(const CkFftContext* c);
 long long transform_bytes(int kind, int log2n);

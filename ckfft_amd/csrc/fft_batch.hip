@@ -5,18 +5,16 @@
 // mix of sizes, kinds and directions (each with its own device buffers) runs as ONE
 // launch of a persistent kernel:
 //  * the host groups the transforms into work items -- a group of same-class transforms
-//    that fills one 512-thread workgroup, or one whole complex 2^16 transform -- and
-//    orders them largest-first;
+//    that fills one 512-thread workgroup -- and orders them largest-first;
 //  * one workgroup per CU loops: take the next item from an atomic queue, run it with
-//    the same transform bodies the batch kernels use (fft_lds.h, fft_pass.h);
-//  * a complex 2^16 transform runs as its two passes inside one workgroup: 8 first-pass
-//    tiles into a scratch buffer, a block barrier (which makes the block's global writes
-//    visible to the block), 8 second-pass tiles.  Items never wait on each other, so
-//    the kernel needs no cross-workgroup synchronisation and no cache flushes;
+//    the same transform body the batch kernels use (fft_lds.h).  Items never wait on
+//    each other, so the kernel needs no cross-workgroup synchronisation;
 //  * the last workgroup to leave resets the queue, so a plan re-executes (eagerly or as
 //    a replayed hipGraph) with no other launch.
-// Transforms the persistent kernel does not take (closed-form sizes, sizes above 2^16)
-// run through the ordinary launches after it, on the same stream.
+// Complex 2^16 transforms (above one workgroup's LDS) run after it as two table-driven
+// pass launches per direction over all of them, with the tiles of the stand-alone
+// multi-pass kernels; transforms neither takes (closed-form sizes, sizes above 2^16) run
+// through the ordinary launches, on the same stream.
 //
 // Autotuning: every class (kind, size) may run with 16 or 32 points per thread, which
 // changes waves per transform, transforms per workgroup and LDS tile, but never the
@@ -39,9 +37,7 @@
 namespace ckfft_gpu {
 
 constexpr int kPersistLdsFloats = 34816;        // 139264 bytes
-constexpr int kClsPass = 4096;                  // + inverse
-constexpr int kPassLogR = 8, kPassLogCB = 5;
-constexpr int kPassTiles = (1 << (kPassLogN - kPassLogR)) >> kPassLogCB;
+constexpr int kPassLogR = 8;                    // radix of both passes of a complex 2^16
 
 __host__ __device__ constexpr int persist_logp(int logc, int var)
 {
@@ -95,49 +91,6 @@ __device__ __forceinline__ void dispatch_lds(int logc, int var, const BatchItem&
     (void)((logc == LO + L ? run_lds_var<KIND, LO + L>(var, w, trs, tw, lds) : 0), ...);
 }
 
-// A complex 2^16 transform: both passes (radix 2^8 each, the same radices as the
-// stand-alone multi-pass path) in this workgroup, tile after tile.
-template <bool INV>
-__device__ __forceinline__ void run_twopass_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
-                                                 const DeviceTables& tb, float* lds)
-{
-    const BatchTransform d = trs[w.first];
-    // opaque thread index, as in run_lds_item: keeps this body's address arithmetic its own
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    __builtin_assume(tid >= 0 && tid < kPersistWG);
-    using PT = PassTile<kPassLogR, kPassLogCB>;
-    static_assert(kPassTiles % 2 == 0, "tiles run in pairs");
-    // Each pass loads tile q+1 into the other register set while tile q computes.
-    float2 va[PT::P], vb[PT::P];
-    pass_tile_load<kPassLogR, kPassLogCB>(d.in, 0, kPassLogN, tid, va);
-#pragma unroll 1
-    for (int q = 0; q < kPassTiles; q += 2) {
-        pass_tile_load<kPassLogR, kPassLogCB>(d.in, q + 1, kPassLogN, tid, vb);
-        pass_tile_run<kPassLogR, kPassLogCB, INV, true>(va, d.scratch, q, kPassLogN, 0, tb.pyramid, tb.fine, tb.xtw,
-                                                        tb.log_xtw, lds, tid);
-        __syncthreads();
-        if (q + 2 < kPassTiles) pass_tile_load<kPassLogR, kPassLogCB>(d.in, q + 2, kPassLogN, tid, va);
-        pass_tile_run<kPassLogR, kPassLogCB, INV, true>(vb, d.scratch, q + 1, kPassLogN, 0, tb.pyramid, tb.fine,
-                                                        tb.xtw, tb.log_xtw, lds, tid);
-        __syncthreads();
-    }
-    // Pass 2 reads what pass 1 wrote: its first load follows the barrier that ends pass 1.
-    // A context that admits 2^16 transforms holds the transposed twiddles of level 2^16.
-    pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, 0, kPassLogN, tid, va);
-#pragma unroll 1
-    for (int q = 0; q < kPassTiles; q += 2) {
-        pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, q + 1, kPassLogN, tid, vb);
-        pass_tile_run<kPassLogR, kPassLogCB, INV, false, 2>(va, d.out, q, kPassLogN, kPassLogR, tb.pyramid, tb.fine,
-                                                            tb.xtw, kPassLogN, lds, tid);
-        __syncthreads();
-        if (q + 2 < kPassTiles) pass_tile_load<kPassLogR, kPassLogCB>(d.scratch, q + 2, kPassLogN, tid, va);
-        pass_tile_run<kPassLogR, kPassLogCB, INV, false, 2>(vb, d.out, q + 1, kPassLogN, kPassLogR, tb.pyramid,
-                                                            tb.fine, tb.xtw, kPassLogN, lds, tid);
-        __syncthreads();
-    }
-}
-
 // The persistent loop, shared by the kernel that reads its queue from device memory and
 // the one that receives a small queue by value in its arguments.
 __device__ __forceinline__ void persistent_body(const BatchItem* __restrict__ items, int n_items,
@@ -154,29 +107,22 @@ __device__ __forceinline__ void persistent_body(const BatchItem* __restrict__ it
         // claim the next item now, so the queue round trip overlaps this item's work
         if (tid == 0) s_item = atomicAdd(ctl, 1);
         const BatchItem w = items[it];
-        if (w.cls >= kClsPass) {
-            if (w.cls == kClsPass)
-                run_twopass_item<false>(w, trs, tb, lds);
-            else
-                run_twopass_item<true>(w, trs, tb, lds);
-        } else {
-            const int var = w.cls & 1;
-            const int kind = (w.cls >> 1) >> 4;
-            const int logc = (w.cls >> 1) & 15;
-            switch (kind) {
-            case kComplexForward:
-                dispatch_lds<kComplexForward, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
-                break;
-            case kComplexInverse:
-                dispatch_lds<kComplexInverse, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
-                break;
-            case kRealForward:
-                dispatch_lds<kRealForward, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
-                break;
-            default:
-                dispatch_lds<kRealInverse, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
-                break;
-            }
+        const int var = w.cls & 1;
+        const int kind = (w.cls >> 1) >> 4;
+        const int logc = (w.cls >> 1) & 15;
+        switch (kind) {
+        case kComplexForward:
+            dispatch_lds<kComplexForward, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+            break;
+        case kComplexInverse:
+            dispatch_lds<kComplexInverse, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+            break;
+        case kRealForward:
+            dispatch_lds<kRealForward, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+            break;
+        default:
+            dispatch_lds<kRealInverse, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+            break;
         }
         __syncthreads();   // the LDS buffer is reused by the next item
     }
@@ -271,7 +217,7 @@ bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int inde
 }
 
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out, bool twopass_as_extra)
+                 bool check_pointers, BatchBuild& out, TwoPass two_pass)
 {
     out = BatchBuild{};
     struct Entry {
@@ -289,12 +235,15 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         // the persistent kernel computes in FP32; transforms of the split-BF16x3 mode run
         // as ordinary launches of that mode's kernels
         if (ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n)) ok = false;
-        // the same two radix-2^8 passes, as separate launches over many workgroups
-        if (twopass_as_extra && !real && log2n == kPassLogN) ok = false;
-        if (ok)
+        const bool is_twopass = ok && !real && log2n == kPassLogN;
+        if (is_twopass && two_pass == TwoPass::kTable) {
+            (d.kind == kComplexInverse ? out.pass_inv : out.pass_fwd)
+                .push_back(BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr});
+        } else if (ok && !is_twopass) {
             pers.push_back({d.kind, log2n, logc, i});
-        else
+        } else {
             out.extras.push_back(BatchExtra{(Kind)d.kind, log2n, d.input, d.output});
+        }
         out.bytes += transform_bytes(d.kind, log2n);
     }
     // group by class, keep descriptor order within a class
@@ -309,14 +258,6 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
     }
     for (size_t i = 0; i < pers.size();) {
         const Entry& e = pers[i];
-        if (e.logc == kPassLogN) {
-            out.twopass.push_back((int)i);
-            const int inv = e.kind == kComplexInverse ? 1 : 0;
-            // both passes move the signal: 4 x 16 bytes per point
-            work.push_back({64ll << kPassLogN, BatchItem{kClsPass + inv, (int)i, 1, 0}});
-            ++i;
-            continue;
-        }
         const int var = var_override >= 0 ? var_override : ctx->tune[e.kind][e.logc];
         const int T = 1 << (e.logc - persist_logp(e.logc, var));
         const int tpw = kPersistWG / T;
@@ -337,9 +278,52 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
     return true;
 }
 
+size_t batch_scratch_transforms(const BatchBuild& b) { return b.pass_fwd.size() + b.pass_inv.size(); }
+
 void batch_assign_scratch(BatchBuild& b, float2* scratch)
 {
-    for (size_t k = 0; k < b.twopass.size(); ++k) b.trs[b.twopass[k]].scratch = scratch + (k << kPassLogN);
+    size_t k = 0;
+    for (auto& t : b.pass_fwd) t.scratch = scratch + (k++ << kPassLogN);
+    for (auto& t : b.pass_inv) t.scratch = scratch + (k++ << kPassLogN);
+}
+
+// Complex 2^16 transforms listed in a table, each with its own input, output and
+// intermediate: one pass over all of them, tiles dealt over the whole GPU.  The same
+// tiles (and so the same results) as the stand-alone multi-pass launches.
+template <bool INV, bool FIRST>
+__global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kernel(
+    const BatchTransform* __restrict__ trs, const float2* __restrict__ pyr, const float2* __restrict__ fine,
+    const float2* __restrict__ xtw, int log_xtw, long long tiles_per_signal, long long total_tiles)
+{
+    using SH = PassShape<kPassLogR>;
+    __shared__ float lds[SH::CB * SH::RSTRIDE];
+    const long long tile = xcd_tile(blockIdx.x, total_tiles);
+    const long long g = tile / tiles_per_signal;
+    const long long q = tile - g * tiles_per_signal;
+    const BatchTransform d = trs[g];
+    pass_tile<kPassLogR, SH::LOGCB, INV, FIRST>(FIRST ? d.in : d.scratch, FIRST ? d.scratch : d.out, q, kPassLogN,
+                                                FIRST ? 0 : kPassLogN - kPassLogR, pyr, fine, xtw, log_xtw, lds,
+                                                threadIdx.x);
+}
+
+hipError_t launch_pass_table(const BatchTransform* d_trs, int count, bool inverse, const DeviceTables& tb,
+                             hipStream_t s)
+{
+    static_assert(2 * kPassLogR == kPassLogN, "two equal passes");
+    if (count <= 0) return hipSuccess;
+    using SH = PassShape<kPassLogR>;
+    const long long tiles = (1ll << (kPassLogN - kPassLogR)) / SH::CB;
+    const long long total = tiles * count;
+    if (total > 0x7fffffffll) return hipErrorInvalidValue;
+    auto k1 = inverse ? fft_pass_table_kernel<true, true> : fft_pass_table_kernel<false, true>;
+    auto k2 = inverse ? fft_pass_table_kernel<true, false> : fft_pass_table_kernel<false, false>;
+    hipLaunchKernelGGL(k1, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, tb.pyramid, tb.fine, tb.xtw, tb.log_xtw,
+                       tiles, total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k2, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, tb.pyramid, tb.fine, tb.xtw, tb.log_xtw,
+                       tiles, total);
+    return hipGetLastError();
 }
 
 int persistent_grid(const CkFftContext* ctx)
@@ -387,6 +371,8 @@ struct _CkFftBatchPlan {
     BatchTransform* d_trs = nullptr;
     int* d_ctl = nullptr;
     float2* d_scratch = nullptr;
+    BatchTransform* d_pass = nullptr;   // table pass transforms: forward ones, then inverse
+    int n_pass_fwd = 0, n_pass_inv = 0;
     std::vector<BatchExtra> extras;
     bool use_graph = false;
     bool graph_ready = false;
@@ -402,17 +388,28 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
 {
     if (!ctx || ctx->magic != kContextMagic || count < 0 || (count > 0 && !descs)) return nullptr;
     BatchBuild b;
-    if (!batch_build(ctx, descs, count, var_override, true, b)) return nullptr;
+    if (!batch_build(ctx, descs, count, var_override, true, b, TwoPass::kTable)) return nullptr;
     auto* plan = new (std::nothrow) CkFftBatchPlan();
     if (!plan) return nullptr;
     plan->ctx = ctx;
     plan->use_graph = (flags & kCkFftBatch_Graph) != 0;
     plan->extras = b.extras;
     plan->n_items = (int)b.items.size();
-    if (!b.twopass.empty() &&
-        !batch_error(hipMalloc(&plan->d_scratch, sizeof(float2) * (b.twopass.size() << kPassLogN)), "hipMalloc(scratch)"))
+    plan->n_pass_fwd = (int)b.pass_fwd.size();
+    plan->n_pass_inv = (int)b.pass_inv.size();
+    if (batch_scratch_transforms(b) &&
+        !batch_error(hipMalloc(&plan->d_scratch, sizeof(float2) * (batch_scratch_transforms(b) << kPassLogN)),
+                     "hipMalloc(scratch)"))
         goto bad;
     batch_assign_scratch(b, plan->d_scratch);
+    if (plan->n_pass_fwd + plan->n_pass_inv) {
+        std::vector<BatchTransform> table(b.pass_fwd);
+        table.insert(table.end(), b.pass_inv.begin(), b.pass_inv.end());
+        if (!batch_error(hipMalloc(&plan->d_pass, sizeof(BatchTransform) * table.size()), "hipMalloc(pass table)") ||
+            !batch_error(hipMemcpy(plan->d_pass, table.data(), sizeof(BatchTransform) * table.size(),
+                                   hipMemcpyHostToDevice), "upload pass table"))
+            goto bad;
+    }
     if (plan->n_items) {
         if (!batch_error(hipMalloc(&plan->d_items, sizeof(BatchItem) * b.items.size()), "hipMalloc(items)") ||
             !batch_error(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * b.trs.size()), "hipMalloc(transforms)") ||
@@ -432,6 +429,7 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
     plan->info.work_items = plan->n_items;
     plan->info.workgroups = plan->grid;
     plan->info.extra_launches = (int)plan->extras.size();
+    plan->info.pass_transforms = plan->n_pass_fwd + plan->n_pass_inv;
     plan->info.algorithmic_bytes = b.bytes;
     return plan;
 bad:
@@ -439,9 +437,15 @@ bad:
     return nullptr;
 }
 
-hipError_t launch_persistent(CkFftBatchPlan* p, hipStream_t s)
+// The plan's launches that a graph can hold: the persistent kernel and the table passes.
+hipError_t launch_plan_kernels(CkFftBatchPlan* p, hipStream_t s)
 {
-    return launch_persistent_raw(p->d_items, p->n_items, p->d_trs, p->d_ctl, p->grid, context_tables(p->ctx), s);
+    const DeviceTables tb = context_tables(p->ctx);
+    hipError_t e = hipSuccess;
+    if (p->n_items) e = launch_persistent_raw(p->d_items, p->n_items, p->d_trs, p->d_ctl, p->grid, tb, s);
+    if (e == hipSuccess) e = launch_pass_table(p->d_pass, p->n_pass_fwd, false, tb, s);
+    if (e == hipSuccess) e = launch_pass_table(p->d_pass + p->n_pass_fwd, p->n_pass_inv, true, tb, s);
+    return e;
 }
 
 }  // namespace
@@ -457,13 +461,13 @@ int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream)
 {
     if (!plan) return 0;
     hipStream_t s = (hipStream_t)stream;
-    if (plan->n_items) {
+    if (plan->n_items + plan->n_pass_fwd + plan->n_pass_inv) {
         if (plan->use_graph) {
             if (!plan->graph_ready) {
                 hipStream_t cap = nullptr;
                 if (!batch_error(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream")) return 0;
                 bool ok = batch_error(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture") &&
-                          batch_error(launch_persistent(plan, cap), "capture launch") &&
+                          batch_error(launch_plan_kernels(plan, cap), "capture launch") &&
                           batch_error(hipStreamEndCapture(cap, &plan->graph), "end capture") &&
                           batch_error(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0), "instantiate");
                 (void)hipStreamDestroy(cap);
@@ -471,7 +475,7 @@ int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream)
                 plan->graph_ready = true;
             }
             if (!batch_error(hipGraphLaunch(plan->gexec, s), "graph launch")) return 0;
-        } else if (!batch_error(launch_persistent(plan, s), "persistent launch")) {
+        } else if (!batch_error(launch_plan_kernels(plan, s), "persistent launch")) {
             return 0;
         }
     }
@@ -497,6 +501,7 @@ void CkFftBatchPlanDestroy(CkFftBatchPlan* plan)
     if (plan->d_trs) (void)hipFree(plan->d_trs);
     if (plan->d_ctl) (void)hipFree(plan->d_ctl);
     if (plan->d_scratch) (void)hipFree(plan->d_scratch);
+    if (plan->d_pass) (void)hipFree(plan->d_pass);
     delete plan;
 }
 

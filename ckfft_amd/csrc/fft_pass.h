@@ -64,32 +64,31 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
 
 // One tile of one pass: butterflies q*CB .. q*CB + CB-1 of the signal at `src` (2^logn
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
-// `lds` holds CB * RSTRIDE floats.  Used by the pass kernel below and by the persistent
-// kernel.  A tile is split into its loads (pass_tile_load: the tile's points into the
-// thread's registers) and the rest (pass_tile_run: inter-pass twiddles, the sub-stages,
-// the stores), so that a caller running several tiles can load the next one while it
-// computes the current one.
+// `lds` holds CB * RSTRIDE floats.  Used by the pass kernels here and by the batch
+// plans' table-driven passes.
 // XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
-// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time;
-// 2 = always from the transposed copy (the caller guarantees it covers the pass).
-template <int LOGR, int LOGCB>
-struct PassTile {
+// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time.
+template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
+__device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
+                                          int logn, int logns, const float2* __restrict__ pyr,
+                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
+                                          int log_xtw, float* lds, int tid)
+{
     using SH = PassShape<LOGR, LOGCB>;
     using PL = Plan<LOGR, SH::LOGP>;
     static_assert(PL::NSTAGES >= 2, "a pass needs at least two sub-stages");
-    static constexpr int P = PL::P;
-};
-
-template <int LOGR, int LOGCB>
-__device__ __forceinline__ void pass_tile_load(const float2* __restrict__ src, long long q, int logn, int tid,
-                                               float2* v)
-{
-    using SH = PassShape<LOGR, LOGCB>;
-    using PL = typename PassTile<LOGR, LOGCB>::PL;
-    constexpr int T = SH::T, CB = SH::CB;
-    const long long stride = (1ll << logn) >> LOGR;          // N/R
+    constexpr int R = SH::R, T = SH::T, CB = SH::CB;
+    const long long stride = (1ll << logn) >> LOGR;         // N/R
     const long long j0 = q * CB;
-    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;   // DFT-fastest lanes
+
+    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
+    const int cB = tid / T, tB = tid - cB * T;             // point-fastest lanes
+    float* reA = lds + cA * SH::RSTRIDE;
+    float* imA = reA + SH::SPAN;
+    float* reB = lds + cB * SH::RSTRIDE;
+    float* imB = reB + SH::SPAN;
+
+    float2 v[PL::P];
     constexpr int R0 = 1 << PL::rlog(0);
     constexpr int B0 = PL::P / R0;
 #pragma unroll
@@ -99,31 +98,10 @@ __device__ __forceinline__ void pass_tile_load(const float2* __restrict__ src, l
             const int r = in_index<PL, 0>(tA + b * T, rr);
             v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
         }
-}
-
-template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
-__device__ __forceinline__ void pass_tile_run(float2* v, float2* __restrict__ dst,
-                                              long long q, int logn, int logns, const float2* __restrict__ pyr,
-                                              const float2* __restrict__ fine, const float2* __restrict__ xtw,
-                                              int log_xtw, float* lds, int tid)
-{
-    using SH = PassShape<LOGR, LOGCB>;
-    using PL = typename PassTile<LOGR, LOGCB>::PL;
-    constexpr int R = SH::R, T = SH::T, CB = SH::CB;
-    const long long j0 = q * CB;
-
-    const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
-    const int cB = tid / T, tB = tid - cB * T;             // point-fastest lanes
-    float* reA = lds + cA * SH::RSTRIDE;
-    float* imA = reA + SH::SPAN;
-    float* reB = lds + cB * SH::RSTRIDE;
-    float* imB = reB + SH::SPAN;
-    constexpr int R0 = 1 << PL::rlog(0);
-    constexpr int B0 = PL::P / R0;
     if constexpr (!FIRST) {
         const long long jm = (j0 + cA) & ((1ll << logns) - 1);
         const int loglev = logns + LOGR;
-        if (XTW == 2 || (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw)) {
+        if (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw) {
             // the same values from the transposed copy: neighbouring columns, neighbouring words
             const float2* tcol = xtw + jm;
             const int sh = log_xtw - loglev;
@@ -134,7 +112,7 @@ __device__ __forceinline__ void pass_tile_run(float2* v, float2* __restrict__ ds
                     const int r = in_index<PL, 0>(tA + b * T, rr);
                     v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
                 }
-        } else if constexpr (XTW != 2) {
+        } else {
 #pragma unroll
             for (int b = 0; b < B0; ++b)
 #pragma unroll
@@ -168,17 +146,6 @@ __device__ __forceinline__ void pass_tile_run(float2* v, float2* __restrict__ ds
             for (int rr = 0; rr < RL; ++rr)
                 base[(long long)out_index<PL, SL>(tA + b * T, rr) * ns] = v[b * RL + rr];
     }
-}
-
-template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
-__device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
-                                          int logn, int logns, const float2* __restrict__ pyr,
-                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
-                                          int log_xtw, float* lds, int tid)
-{
-    float2 v[PassTile<LOGR, LOGCB>::P];
-    pass_tile_load<LOGR, LOGCB>(src, q, logn, tid, v);
-    pass_tile_run<LOGR, LOGCB, INV, FIRST, XTW>(v, dst, q, logn, logns, pyr, fine, xtw, log_xtw, lds, tid);
 }
 
 }  // namespace ckfft_gpu

@@ -10,8 +10,9 @@
 //    uploaded asynchronously from the slot's pinned buffers -- so batches formed while
 //    others run overlap on the device;
 //  * polls the slots' completion events, finishes their requests and frees budget.
-// Twiddles, descriptor buffers, scratch and streams stay resident for the server's
-// life.  A request's result does not depend on the batch it ran in: every kernel that
+// Complex 2^16 requests (above one workgroup's LDS) run as ordinary multi-pass launches
+// in the batch's stream, their intermediate charged to the budget with the request.
+// Twiddles, descriptor buffers and streams stay resident for the server's life.  A request's result does not depend on the batch it ran in: every kernel that
 // computes a transform rounds the same way.
 #include <hip/hip_runtime.h>
 
@@ -152,7 +153,7 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
     // descriptors were validated at submission
     // requests arrive a few at a time: a complex 2^16 one is faster as its own multi-pass
     // launch than as one workgroup's item
-    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b, true)) return false;
+    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b, TwoPass::kExtra)) return false;
     const DeviceTables tb = context_tables(S->ctx);
     bool ok = true;
     if (!b.items.empty() && b.items.size() <= (size_t)kInlineQueue && b.trs.size() <= (size_t)kInlineQueue) {

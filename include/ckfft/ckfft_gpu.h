@@ -39,9 +39,12 @@ int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
  *
  * A batch plan takes any mix of transforms -- each with its own kind, size and device
  * buffers -- and runs them as one launch of a persistent kernel (workgroups pull work
- * items from an on-device queue, largest first).  Transforms the persistent kernel does
- * not take (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches
- * after it on the same stream; CkFftBatchInfo reports how many.
+ * items from an on-device queue, largest first).  Complex 2^16 transforms run as two
+ * table-driven pass launches per direction over all of them (every workgroup of the GPU
+ * on one transform's tiles at a time), and transforms the persistent kernel does not take
+ * (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches; all on the same
+ * stream after the persistent launch.  CkFftBatchInfo reports how many transforms each
+ * route takes.
  * Results are bit-identical to the single-signal calls for every transform.
  * Executions of one plan must not overlap (they share its queue).  The reference has
  * no counterpart for these entry points. */
@@ -67,6 +70,7 @@ typedef struct {
     int workgroups;             /* persistent grid size */
     int extra_launches;         /* transforms run as ordinary launches */
     long long algorithmic_bytes;/* one read of every input, one write of every output */
+    int pass_transforms;        /* complex 2^16 run by the table-driven pass launches */
 } CkFftBatchInfo;
 
 typedef struct _CkFftBatchPlan CkFftBatchPlan;

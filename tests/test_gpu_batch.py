@@ -91,6 +91,8 @@ def test_uneven_batch_equals_single_calls(torch, C, ctx):
     info = plan.info()
     assert info["transforms"] == len(mix)
     assert info["extra_launches"] >= 3          # the closed-form sizes and 2^17
+    # complex 2^16: table-driven pass launches
+    assert info["pass_transforms"] == sum(k in ("cfwd", "cinv") and n == 1 << 16 for k, n in mix)
     assert plan.execute() == 1
     torch.cuda.synchronize()
     for i, (kind, n) in enumerate(mix):
@@ -184,7 +186,9 @@ def test_config_uneven_sample_2000(torch, C, ctx):
         mix.append((kind, 1 << k))
     ins, outs, descs = build(torch, C, ctx, mix, 5000)
     plan = C.BatchPlan(ctx, descs, graph=True)
-    assert plan.info()["extra_launches"] == 0
+    info = plan.info()
+    assert info["extra_launches"] == 0 and info["pass_transforms"] > 0
+    assert info["persistent_transforms"] + info["pass_transforms"] == len(mix)
     assert plan.execute() == 1
     torch.cuda.synchronize()
     for i in range(0, 2000, 7):
