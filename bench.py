@@ -473,15 +473,9 @@ def main():
     ctx = C.Context(n, C.kCkFftDirection_Both)
     stream = torch.cuda.current_stream(dev)
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record(stream)
+    def step():
         assert ctx.real_forward_batch(n, B, x, y, stream) == 1
-        if ev is not None:
-            ev[1].record(stream)
         assert ctx.real_inverse_batch(n, B, y, z, stream) == 1
-        if ev is not None:
-            ev[2].record(stream)
 
     for _ in range(args.warmup):
         step()
@@ -499,16 +493,22 @@ def main():
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
     ms_per_step = total_ms / args.steps
-    # per-launch durations for the roofline: the same K steps again, each launch between
-    # its own pair of events on the launch stream (the events add a small gap, so these
-    # durations are upper bounds of the kernels' time in the step above)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    inv_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    # per-launch durations for the roofline: each kernel K times back to back between two
+    # events on the launch stream, elapsed / K (the dispatch gaps between the launches are
+    # included, so these are upper bounds of the kernels' own durations)
+    def back_to_back(launch):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(args.steps):
+            launch()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    fwd_ms = back_to_back(lambda: ctx.real_forward_batch(n, B, x, y, stream))
+    inv_ms = back_to_back(lambda: ctx.real_inverse_batch(n, B, y, z, stream))
 
     flops_step = B * 2 * real_flops(n)
     value = flops_step / (ms_per_step * 1e-3) / 1e9
@@ -524,7 +524,7 @@ def main():
         "traffic": (traffic or {}).get(dom_name),
         "algorithmic_bytes_per_launch": bytes_launch,
         "launch_ms": {"real_forward": round(fwd_ms, 5), "real_inverse": round(inv_ms, 5)},
-        "launch_timing": "mean over K event-bracketed launches (a second pass of the K steps)",
+        "launch_timing": "K launches of each kernel back to back between two events, elapsed / K",
         "step_GBs": round(2 * bytes_launch / (ms_per_step * 1e-3) / 1e9, 1),
         "frac_per_launch": {"real_forward": round(bytes_launch / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "real_inverse": round(bytes_launch / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},

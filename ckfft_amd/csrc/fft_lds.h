@@ -55,14 +55,15 @@ constexpr int mirror_stage()
     }
 }
 
-template <class PL, int S, bool INV, bool SPLIT, int MS>
+// Stages S .. END-1, each after the exchange that brings its inputs.
+template <class PL, int S, int END, bool INV, bool SPLIT, int MS>
 __device__ __forceinline__ void run_stages(float2* v, int t, float* re, float* im,
                                            const float2* __restrict__ tw)
 {
-    if constexpr (S < PL::NSTAGES) {
+    if constexpr (S < END) {
         exchange<PL, S - 1, S, SPLIT, MS == S - 1, MS == S>(v, t, re, im);
         stage_compute<PL, S, INV, MS == S>(v, t, tw);
-        run_stages<PL, S + 1, INV, SPLIT, MS>(v, t, re, im, tw);
+        run_stages<PL, S + 1, END, INV, SPLIT, MS>(v, t, re, im, tw);
     }
 }
 
@@ -132,14 +133,17 @@ __device__ __forceinline__ void load_first_unpack(float2* v, int t, const float2
     }
 }
 
-// Real inverse with a mirror-closed first stage: each Y value is read once, by the
-// thread that also holds its mirror, and the pairs are unpacked in registers.  Element
-// (b, r) of a lower slot b < H is point k = jl + r*M and pairs with element (b+H, R-1-r),
-// point N - k.  Thread 0's slot pair holds the self-mirrored butterflies 0 and M/2
-// instead; it fetches its partners from memory (predicated loads in one lane).
-template <class PL>
-__device__ __forceinline__ void load_first_unpack_mirror(float2* v, int t, const float2* __restrict__ y,
-                                                         const float2* __restrict__ tw)
+// Real inverse with a mirror-closed first stage, fused with that stage: each Y value is
+// read once, by the thread that also holds its mirror, and the pairs are unpacked in
+// registers.  Element (b, r) of a lower slot b < H is point k = jl + r*M and pairs with
+// element (b+H, R-1-r), point N - k.  Thread 0's slot pair holds the self-mirrored
+// butterflies 0 and M/2 instead; it fetches its partners from memory (loads in one lane).
+// The loads are issued pair by pair (slots b and b+H, then thread 0's partners after the
+// first pair), ahead of the arithmetic; each pair is then unpacked and its two
+// butterflies computed.
+template <class PL, bool INV>
+__device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, const float2* __restrict__ y,
+                                                          const float2* __restrict__ tw)
 {
     constexpr int N = PL::N;
     constexpr int R = 1 << PL::rlog(0);
@@ -147,35 +151,48 @@ __device__ __forceinline__ void load_first_unpack_mirror(float2* v, int t, const
     constexpr int H = B / 2;
     constexpr int M = N / R;
     const float2* lvl = tw + 2 * N;
+    float2 q0[R], q1[R];                                // thread 0: partners of its first pair
+    float2 wl[H];                                       // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
 #pragma unroll
-    for (int b = 0; b < B; ++b)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b), r)];
-    const int jh0 = bfly<PL, 0, true>(t, H);            // M - t, or M/2 for t == 0
+    for (int b = 0; b < H; ++b) wl[b] = lvl[t + b * PL::T];
 #pragma unroll
     for (int b = 0; b < H; ++b) {
-        const int jl = t + b * PL::T;
-        const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wl = lvl[jl];                       // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[b * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b), r)];
+            v[(b + H) * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b + H), r)];
+        }
+        if (b == 0 && t == 0) {
+            // butterflies 0 and M/2: point r*M pairs with N - r*M, point M/2 + (R-1-r)*M
+            // with M/2 + r*M
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                q0[r] = y[N - r * M];
+                q1[r] = y[M / 2 + r * M];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < H; ++b) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             float2& lo = v[b * R + r];
             float2& hi = v[(b + H) * R + R - 1 - r];
-            const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
-            const float2 w0 = cmul(wl, w64(r * (32 / R)));
+            const float2 w0 = cmul(wl[b], w64(r * (32 / R)));
             float2 p0 = hi, p1 = lo;                     // Y[N - k0], Y[N - k1] when paired
             float2 w1 = make_float2(-w0.x, w0.y);        // W^(N-k0) = -conj(W^k0)
             if (b == 0 && t == 0) {
-                // butterflies 0 and M/2 of thread 0: partners from memory, and
                 // W^(M/2 + (R-1-r)M) = exp(-2 pi i (2(R-1-r)+1) / 4R)
-                p0 = y[N - k0];
-                p1 = y[N - k1];
+                p0 = q0[r];
+                p1 = q1[r];
                 w1 = w64((2 * (R - 1 - r) + 1) * (16 / R));
             }
             const float2 y0 = lo, y1 = hi;
             lo = unpack_one(y0, p0, w0);
             hi = unpack_one(y1, p1, w1);
         }
+        butterfly_compute<PL, 0, INV, true>(v, t, tw, b);
+        butterfly_compute<PL, 0, INV, true>(v, t, tw, b + H);
     }
 }
 
@@ -214,13 +231,18 @@ __device__ __forceinline__ void pack_pair(int k, float2 zk, float2 zm, const flo
     if (k != N - k) dst[N - k] = pack_one(zm, zk, edge_twiddle<N, R>(lvl, N - k));
 }
 
-// Real forward epilogue with a mirror-closed last stage: every output point k is written
-// once, by the thread that holds Z[k], from Z[k] and its mirror Z[N-k] in the same
-// thread's registers.  Thread 0's first slot pair holds the self-mirrored butterflies 0
-// and M/2, whose partners sit at other register positions; it also writes out[N].
-template <class PL>
-__device__ __forceinline__ void pack_store_mirror(const float2* v, int t, const float2* __restrict__ tw,
-                                                  float2* __restrict__ dst, bool active)
+// Real forward: the mirror-closed last stage fused with the pack.  Every output point k
+// is written once, by the thread that holds Z[k], from Z[k] and its mirror Z[N-k] in the
+// same thread's registers.  Thread 0's first slot pair holds the self-mirrored
+// butterflies 0 and M/2, whose partners sit at other register positions; it also writes
+// out[N].  Pair by pair (slots b and b+H): the two butterflies compute, then their
+// outputs are packed and stored.  The stage's twiddles (tws) and the pack twiddles W^jl
+// of the lower slots (wl) are read beforehand.
+template <class PL, bool INV>
+__device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const float2* __restrict__ tw,
+                                                       StageTwiddles<PL, PL::NSTAGES - 1, INV, true>& tws,
+                                                       float2 (&wl)[PL::P / (1 << PL::rlog(PL::NSTAGES - 1)) / 2],
+                                                       float2* __restrict__ dst, bool active)
 {
     constexpr int S = PL::NSTAGES - 1;
     constexpr int N = PL::N;
@@ -229,19 +251,22 @@ __device__ __forceinline__ void pack_store_mirror(const float2* v, int t, const 
     constexpr int H = B / 2;
     constexpr int M = N / R;                    // the last stage writes point j + r*M
     const float2* lvl = tw + 2 * N;
-    if (!active) return;
     const int jh0 = bfly<PL, S, true>(t, H);    // M - t, or M/2 for t == 0
+    tws.settle();
+    settle_reads(wl);
 #pragma unroll
     for (int b = 0; b < H; ++b) {
+        butterfly_compute<PL, S, INV, true>(v, tws, b);
+        butterfly_compute<PL, S, INV, true>(v, tws, b + H);
+        if (!active) continue;
         const int jl = t + b * PL::T;
         const int jh = b == 0 ? jh0 : M - jl;
-        const float2 wl = lvl[jl];               // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float2 lo = v[b * R + r];
             const float2 hi = v[(b + H) * R + R - 1 - r];
             const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
-            const float2 w0 = cmul(wl, w64(r * (32 / R)));
+            const float2 w0 = cmul(wl[b], w64(r * (32 / R)));   // W^(jl + rM) = W^jl exp(-2 pi i r/2R)
             float2 p0 = hi, p1 = lo;
             float2 w1 = make_float2(-w0.x, w0.y);   // W^(N-k0) = -conj(W^k0)
             if (b == 0) {
@@ -256,7 +281,29 @@ __device__ __forceinline__ void pack_store_mirror(const float2* v, int t, const 
             dst[k1] = pack_one(hi, p1, w1);
         }
     }
-    if (t == 0) dst[N] = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
+    if (active && t == 0) dst[N] = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
+}
+
+// The last stage of a complex or real-inverse transform fused with its stores, butterfly
+// by butterfly.  The stage's twiddles (tws) are read beforehand: a wait for a read issued
+// after the first stores would also wait for those stores.
+template <class PL, bool INV>
+__device__ __forceinline__ void last_stage_store(float2* v, int t,
+                                                 StageTwiddles<PL, PL::NSTAGES - 1, INV, false>& tws,
+                                                 float2* __restrict__ dst, bool active)
+{
+    constexpr int S = PL::NSTAGES - 1;
+    constexpr int R = 1 << PL::rlog(S);
+    constexpr int B = PL::P / R;
+    tws.settle();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        butterfly_compute<PL, S, INV, false>(v, tws, b);
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
+        }
+    }
 }
 
 // Real forward epilogue when the last stage cannot be mirror-closed: Z goes through
@@ -270,6 +317,7 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
     constexpr int T = PL::T;
     constexpr int S = PL::NSTAGES - 1;
     constexpr int Q = (N / 2) / T + 1;                      // pair slots per thread
+    using LY = ExchangeLayout<PL, S>;                       // the layout stage S writes
     const float2* lvl = tw + 2 * N;
     float2 zk[Q], zm[Q];
     if constexpr (!SPLIT) {
@@ -279,7 +327,7 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
             if (k <= N / 2) {
-                const int ik = lds_pad(k & (N - 1)), im_ = lds_pad((N - k) & (N - 1));
+                const int ik = LY::pad(k & (N - 1)), im_ = LY::pad((N - k) & (N - 1));
                 zk[q] = make_float2(re[ik], im[ik]);
                 zm[q] = make_float2(re[im_], im[im_]);
             }
@@ -291,8 +339,8 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
             if (k <= N / 2) {
-                zk[q].x = re[lds_pad(k & (N - 1))];
-                zm[q].x = re[lds_pad((N - k) & (N - 1))];
+                zk[q].x = re[LY::pad(k & (N - 1))];
+                zm[q].x = re[LY::pad((N - k) & (N - 1))];
             }
         }
         __syncthreads();
@@ -302,8 +350,8 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
             if (k <= N / 2) {
-                zk[q].y = im[lds_pad(k & (N - 1))];
-                zm[q].y = im[lds_pad((N - k) & (N - 1))];
+                zk[q].y = im[LY::pad(k & (N - 1))];
+                zm[q].y = im[LY::pad((N - k) & (N - 1))];
             }
         }
     }
@@ -331,7 +379,7 @@ struct LdsShape {
     static constexpr int TPW = WG / T;                // transforms per workgroup
     static_assert(TPW >= 1 && TPW * T == WG, "workgroup must hold whole transforms");
     static_assert(!SPLIT || TPW == 1, "plane-by-plane exchange is for one transform per workgroup");
-    static constexpr int SPAN = lds_span(N);
+    static constexpr int SPAN = plan_span<PL>();
     static constexpr int LDS_WORDS = T > 1 ? (SPLIT ? SPAN : 2 * SPAN * TPW) : 1;
     static constexpr int MS = mirror_stage<PL, KIND>();
     // complex elements per signal on each side
@@ -361,34 +409,64 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
     float* im = SPLIT ? lds : re + SPAN;
 
     float2 v[PL::P];
-    if constexpr (KIND == kRealInverse) {
-        if constexpr (MS == 0)
-            load_first_unpack_mirror<PL>(v, t, src, tw);
-        else
+    if constexpr (T == 1) {
+        // whole transform in this thread's registers (one stage for n <= 16 points)
+        if constexpr (KIND == kRealInverse)
             load_first_unpack<PL>(v, t, src, tw);
-    } else {
-        load_first_complex<PL>(v, t, src);
-    }
-
-    stage_compute<PL, 0, INV, MS == 0>(v, t, tw);
-    run_stages<PL, 1, INV, SPLIT, MS>(v, t, re, im, tw);
-
-    if constexpr (KIND == kRealForward) {
-        if constexpr (T == 1) {
-            // whole transform in this thread's registers, in natural order
+        else
+            load_first_complex<PL>(v, t, src);
+        stage_compute<PL, 0, INV>(v, t, tw);
+        run_stages<PL, 1, PL::NSTAGES, INV, SPLIT, -1>(v, t, re, im, tw);
+        if constexpr (KIND == kRealForward) {
             if (active) {
                 const float2* lvl = tw + 2 * N;
 #pragma unroll
                 for (int k = 0; k <= N / 2; ++k)
                     pack_pair<N, N>(k, v[k & (N - 1)], v[(N - k) & (N - 1)], lvl, dst);
             }
-        } else if constexpr (MS == PL::NSTAGES - 1) {
-            pack_store_mirror<PL>(v, t, tw, dst, active);
         } else {
-            pack_epilogue<PL, SPLIT>(v, t, re, im, tw, dst, active);
+            store_last<PL>(v, t, dst, active);
         }
     } else {
-        store_last<PL>(v, t, dst, active);
+        static_assert(PL::NSTAGES >= 2, "a transform over several threads has at least two stages");
+        constexpr int SL = PL::NSTAGES - 1;
+        // first stage, fused with the loads (and the real-inverse unpack)
+        if constexpr (KIND == kRealInverse && MS == 0) {
+            first_stage_unpack_mirror<PL, INV>(v, t, src, tw);
+        } else if constexpr (KIND == kRealInverse) {
+            load_first_unpack<PL>(v, t, src, tw);
+            stage_compute<PL, 0, INV>(v, t, tw);
+        } else {
+            load_first_complex<PL>(v, t, src);
+            stage_compute<PL, 0, INV>(v, t, tw);
+        }
+        // middle stages, then the exchange into the last one
+        run_stages<PL, 1, SL, INV, SPLIT, MS>(v, t, re, im, tw);
+        // last stage, fused with the stores (and the real-forward pack); its twiddle reads
+        // are issued before the exchange that brings its inputs
+        if constexpr (KIND == kRealForward && MS == SL) {
+            constexpr int H = bflies<PL, SL>() / 2;
+            StageTwiddles<PL, SL, INV, true> tws(t, tw);
+            float2 wl[H];
+#pragma unroll
+            for (int b = 0; b < H; ++b) wl[b] = tw[2 * N + t + b * T];
+            exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, true>(v, t, re, im);
+            last_stage_pack_mirror<PL, INV>(v, t, tw, tws, wl, dst, active);
+        } else if constexpr (KIND == kRealForward) {
+            exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
+            stage_compute<PL, SL, INV>(v, t, tw);
+            pack_epilogue<PL, SPLIT>(v, t, re, im, tw, dst, active);
+        } else if constexpr (T < 32) {
+            // several transforms per half-wave (small sizes): reading the twiddles ahead
+            // measured slower here
+            exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
+            stage_compute<PL, SL, INV>(v, t, tw);
+            store_last<PL>(v, t, dst, active);
+        } else {
+            StageTwiddles<PL, SL, INV, false> tws(t, tw);
+            exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
+            last_stage_store<PL, INV>(v, t, tws, dst, active);
+        }
     }
 }
 

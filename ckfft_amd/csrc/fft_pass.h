@@ -37,7 +37,7 @@ struct PassShape {
     static constexpr int LOGCB = LOGCB_;
     static constexpr int CB = 1 << LOGCB;
     static constexpr int WG = CB * T;
-    static constexpr int SPAN = lds_span(R);
+    static constexpr int SPAN = plan_span<Plan<LOGR, LOGP>>();
     static constexpr int RSTRIDE = 2 * SPAN + 1;   // per-DFT LDS region; odd so regions start on different banks
 };
 

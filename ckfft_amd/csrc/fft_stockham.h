@@ -5,16 +5,12 @@
 // registers.  Stage s has radix R (16, except one smaller "odd" stage) and span Ns:
 // butterfly j reads point j + r*N/R and writes (j/Ns)*Ns*R + j%Ns + r*Ns (Stockham
 // autosort), so input and output are in natural order.  Between stages the data moves
-// through LDS as separate real/imaginary planes padded by one word per 16.
+// through LDS as separate real/imaginary planes, padded per exchange (ExchangeLayout).
 #pragma once
 #include "ckfft_internal.h"
 #include "fft_device.h"
 
 namespace ckfft_gpu {
-
-// Padded LDS index: one unused word after every 16.
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
-constexpr int lds_span(int n) { return n + (n >> 4); }
 
 // Radix plan: every stage is radix 2^RMAX_LOG except one smaller "odd" stage, placed
 // first (SMALL_LAST = false) or last (SMALL_LAST = true).
@@ -44,6 +40,37 @@ struct Plan {
 template <class PL, int S>
 constexpr int bflies() { return PL::P >> PL::rlog(S); }
 
+// LDS layout of the exchange that follows stage SW: logical index i is stored at word
+// i + PAD * (i >> K) (a signed i shifts arithmetically).  A half-wave (32 lanes) addresses
+// the 32 banks in one cycle, so the layout is chosen from the accesses of that exchange.
+//  * A transform of >= 32 threads: K = 5.  The next stage reads runs of 32 consecutive
+//    points from 32-aligned starts, which a pad every 32 words never splits.  Stage SW
+//    writes runs of NS consecutive points NS*R apart: no pad when NS >= 32 (the runs are
+//    whole rows), one word per 32 when NS*R < 32 (lane stride R), and otherwise 32/R words,
+//    which moves each of the 32/NS runs of a half-wave by NS banks.
+//  * Fewer threads (several transforms in a half-wave): one word per 16.
+template <class PL, int SW>
+struct ExchangeLayout {
+    static constexpr int R = 1 << PL::rlog(SW);
+    static constexpr int NS = 1 << PL::nslog(SW);
+    static constexpr int K = PL::T >= 32 ? 5 : 4;
+    static constexpr int PAD = PL::T >= 32 ? (NS >= 32 ? 0 : (NS * R < 32 ? 1 : 32 / R)) : 1;
+    __host__ __device__ static constexpr int pad(int i) { return i + PAD * (i >> K); }
+    static constexpr int SPAN = (PL::N - 1) + PAD * ((PL::N - 1) >> K) + 1;   // words per plane
+};
+
+// Words per LDS plane that every exchange of a plan fits in.
+template <class PL, int S = 0>
+constexpr int plan_span()
+{
+    if constexpr (S + 1 >= PL::NSTAGES) {
+        return ExchangeLayout<PL, S>::SPAN;
+    } else {
+        constexpr int a = ExchangeLayout<PL, S>::SPAN, b = plan_span<PL, S + 1>();
+        return a > b ? a : b;
+    }
+}
+
 // Thread -> butterfly assignment of stage S.  Normally thread t owns butterflies
 // t + b*T.  In a "mirror-closed" stage (MIRROR, needs an even count B >= 2) thread t owns
 // j = t + b*T for b < B/2 and M - j for the other half (M = N/R butterflies; thread 0
@@ -69,29 +96,47 @@ __device__ __forceinline__ int bfly(int t, int b)
 
 // Multiply a[1..R-1] by w^r, w = lvl[k].  Only the power-of-two powers w, w^2, w^4, w^8
 // are read from the table (exact FP64-rounded values); the others are products of at
-// most four of them, which keeps the twiddle working set per butterfly small.
+// most four of them, which keeps the twiddle working set per butterfly small.  The reads
+// (twiddle_load) are separate from the products (twiddle_apply) so that a stage can
+// issue its reads early.
+template <int R>
+struct TwBase {
+    float2 w[R >= 16 ? 4 : R >= 8 ? 3 : R >= 4 ? 2 : 1];   // w, w^2, w^4, w^8
+};
+
 template <int R, bool INV>
-__device__ __forceinline__ void twiddle_mul(float2* a, const float2* __restrict__ lvl, int k)
+__device__ __forceinline__ TwBase<R> twiddle_load(const float2* __restrict__ lvl, int k)
+{
+    TwBase<R> b;
+    b.w[0] = dirw<INV>(lvl[k]);
+    if constexpr (R >= 4) b.w[1] = dirw<INV>(lvl[2 * k]);
+    if constexpr (R >= 8) b.w[2] = dirw<INV>(lvl[4 * k]);
+    if constexpr (R >= 16) b.w[3] = dirw<INV>(lvl[8 * k]);
+    return b;
+}
+
+template <int R>
+__device__ __forceinline__ void twiddle_apply(float2* a, const TwBase<R>& tb)
 {
     if constexpr (R >= 2) {
-        const float2 w1 = dirw<INV>(lvl[k]);
+        const float2 w1 = tb.w[0];
         if constexpr (R == 2) {
             a[1] = cmul(a[1], w1);
         } else {
-            const float2 w2 = dirw<INV>(lvl[2 * k]);
+            const float2 w2 = tb.w[1];
             const float2 w3 = cmul(w1, w2);
             a[1] = cmul(a[1], w1);
             a[2] = cmul(a[2], w2);
             a[3] = cmul(a[3], w3);
             if constexpr (R >= 8) {
-                const float2 w4 = dirw<INV>(lvl[4 * k]);
+                const float2 w4 = tb.w[2];
                 a[4] = cmul(a[4], w4);
                 a[5] = cmul(a[5], cmul(w1, w4));
                 a[6] = cmul(a[6], cmul(w2, w4));
                 const float2 w7 = cmul(w3, w4);
                 a[7] = cmul(a[7], w7);
                 if constexpr (R == 16) {
-                    const float2 w8 = dirw<INV>(lvl[8 * k]);
+                    const float2 w8 = tb.w[3];
                     a[8] = cmul(a[8], w8);
                     a[9] = cmul(a[9], cmul(w1, w8));
                     a[10] = cmul(a[10], cmul(w2, w8));
@@ -107,20 +152,74 @@ __device__ __forceinline__ void twiddle_mul(float2* a, const float2* __restrict_
     }
 }
 
+template <int R, bool INV>
+__device__ __forceinline__ void twiddle_mul(float2* a, const float2* __restrict__ lvl, int k)
+{
+    twiddle_apply<R>(a, twiddle_load<R, INV>(lvl, k));
+}
+
+// The twiddle reads of every butterfly of stage S (stages after the first), issued ahead
+// of the stage so that waiting for them never waits for later memory operations.
 template <class PL, int S, bool INV, bool MIRROR = false>
-__device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw)
+struct StageTwiddles {
+    static constexpr int R = 1 << PL::rlog(S);
+    static constexpr int NS = 1 << PL::nslog(S);
+    TwBase<R> b[PL::P / R];
+    __device__ __forceinline__ StageTwiddles(int t, const float2* __restrict__ tw)
+    {
+        static_assert(NS > 1, "the first stage has no twiddles");
+#pragma unroll
+        for (int i = 0; i < PL::P / R; ++i)
+            b[i] = twiddle_load<R, INV>(tw + NS * R, bfly<PL, S, MIRROR>(t, i) & (NS - 1));
+    }
+    // Have every read land here.  A stage that stores under a branch needs this before its
+    // first store: the compiler's wait for a later read, placed after stores it cannot
+    // prove were issued, would otherwise count those stores and wait for them as well.
+    __device__ __forceinline__ void settle()
+    {
+#pragma unroll
+        for (int i = 0; i < PL::P / R; ++i)
+#pragma unroll
+            for (int j = 0; j < (int)(sizeof(b[i].w) / sizeof(float2)); ++j)
+                asm volatile("" : "+v"(b[i].w[j].x), "+v"(b[i].w[j].y));
+    }
+};
+
+// The same for a plain array of values read from memory.
+template <int K>
+__device__ __forceinline__ void settle_reads(float2 (&w)[K])
+{
+#pragma unroll
+    for (int i = 0; i < K; ++i) asm volatile("" : "+v"(w[i].x), "+v"(w[i].y));
+}
+
+// Butterfly b of stage S with twiddles read beforehand.
+template <class PL, int S, bool INV, bool MIRROR>
+__device__ __forceinline__ void butterfly_compute(float2* v, const StageTwiddles<PL, S, INV, MIRROR>& tws, int b)
+{
+    constexpr int R = 1 << PL::rlog(S);
+    twiddle_apply<R>(v + b * R, tws.b[b]);
+    dft<R, INV>(v + b * R);
+}
+
+// Butterfly b of stage S: its twiddles, then its DFT.
+template <class PL, int S, bool INV, bool MIRROR = false>
+__device__ __forceinline__ void butterfly_compute(float2* v, int t, const float2* __restrict__ tw, int b)
 {
     constexpr int R = 1 << PL::rlog(S);
     constexpr int NS = 1 << PL::nslog(S);
-    constexpr int B = PL::P / R;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        if constexpr (NS > 1) {
-            const int k = bfly<PL, S, MIRROR>(t, b) & (NS - 1);
-            twiddle_mul<R, INV>(v + b * R, tw + NS * R, k);     // level NS*R of the pyramid
-        }
-        dft<R, INV>(v + b * R);
+    if constexpr (NS > 1) {
+        const int k = bfly<PL, S, MIRROR>(t, b) & (NS - 1);
+        twiddle_mul<R, INV>(v + b * R, tw + NS * R, k);     // level NS*R of the pyramid
     }
+    dft<R, INV>(v + b * R);
+}
+
+template <class PL, int S, bool INV, bool MIRROR = false>
+__device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw)
+{
+#pragma unroll
+    for (int b = 0; b < bflies<PL, S>(); ++b) butterfly_compute<PL, S, INV, MIRROR>(v, t, tw, b);
 }
 
 // Index of element r of butterfly j at the input of stage S.
@@ -145,15 +244,15 @@ __device__ __forceinline__ int out_index(int j, int r)
 // splits as U(t, g) + C(b, r): U depends on the thread and on the butterfly's group g
 // (0: j = t + bT; in a mirror-closed stage 1: the first mirrored butterfly, where thread
 // 0 takes M/2, and 2: the other mirrored ones), C is a compile-time constant.  The low
-// four bits of U and C never carry into each other, so the padded index is
-// pad(U) + pad(C): each thread computes at most three bases once and every access is an
-// immediate offset.  ok() proves the split at compile time; the stages where it cannot
-// (transforms of fewer than 16 threads) use the plain formula.
-
-__host__ __device__ constexpr int pad_any(int x) { return x + (x >> 4); }   // lds_pad, any sign
+// K bits of U and C (K of the exchange's layout) never carry into each other, so the
+// padded index is pad(U) + pad(C): each thread computes at most three bases once and
+// every access is an immediate offset.  ok() proves the split at compile time; the stages
+// where it cannot (transforms of few threads) use the plain formula.
 
 template <class PL, int S, bool MIRROR, bool OUT>
 struct IndexSplit {
+    // writes of stage S use the layout after S, reads of stage S the layout after S-1
+    using LY = ExchangeLayout<PL, OUT ? S : (S > 0 ? S - 1 : 0)>;
     static constexpr int R = 1 << PL::rlog(S);
     static constexpr int NS = 1 << PL::nslog(S);
     static constexpr int B = PL::P / R;
@@ -191,11 +290,12 @@ struct IndexSplit {
     }
     // U + C equals the index by construction (T and NS are powers of two, one divides the
     // other; a sample of threads re-checks it).  pad(U + C) = pad(U) + pad(C) exactly when
-    // (U & 15) + (C & 15) < 16 (floor division by 16 on both sides), so the bound below,
-    // the largest low part of U over all threads plus that of C over the group's
-    // elements, makes the split exact for every t, b and r.
+    // (U & L) + (C & L) <= L, L = 2^K - 1 (floor division by 2^K on both sides), or when
+    // the layout has no pad; the bound below, the largest low part of U over all threads
+    // plus that of C over the group's elements, makes the split exact for every t, b, r.
     static constexpr bool ok()
     {
+        constexpr int L = (1 << LY::K) - 1;
         const int probe[6] = {0, 1, 15, 16, T / 2, T - 1};
         for (int k = 0; k < 6; ++k) {
             const int t = probe[k] < T ? probe[k] : T - 1;
@@ -207,10 +307,10 @@ struct IndexSplit {
             int mu = -1, mc = -1;
             for (int b = 0; b < B; ++b)
                 if (group(b) == g)
-                    for (int r = 0; r < R; ++r) mc = (C(b, r) & 15) > mc ? (C(b, r) & 15) : mc;
+                    for (int r = 0; r < R; ++r) mc = (C(b, r) & L) > mc ? (C(b, r) & L) : mc;
             if (mc < 0) continue;                                   // no butterfly in this group
-            for (int t = 0; t < T; ++t) mu = (U(t, g) & 15) > mu ? (U(t, g) & 15) : mu;
-            if (mu + mc >= 16) return false;
+            for (int t = 0; t < T; ++t) mu = (U(t, g) & L) > mu ? (U(t, g) & L) : mu;
+            if (LY::PAD != 0 && mu + mc > L) return false;
         }
         return true;
     }
@@ -249,8 +349,9 @@ __device__ __forceinline__ void lds_write(const float2* v, int t, float* re, flo
     using SP = IndexSplit<PL, S, MIRROR, true>;
     constexpr int R = SP::R, B = SP::B;
     if constexpr (kIndexSplits<PL, S, MIRROR, true>) {
-        constexpr int NW = lds_span(PL::N) / kLdsWindow + 1;
-        const int ub[3] = {pad_any(SP::U(t, 0)), MIRROR ? pad_any(SP::U(t, 1)) : 0, MIRROR ? pad_any(SP::U(t, 2)) : 0};
+        constexpr int NW = plan_span<PL>() / kLdsWindow + 1;
+        using LY = typename SP::LY;
+        const int ub[3] = {LY::pad(SP::U(t, 0)), MIRROR ? LY::pad(SP::U(t, 1)) : 0, MIRROR ? LY::pad(SP::U(t, 2)) : 0};
 #pragma unroll
         for (int g = 0; g < (MIRROR ? 3 : 1); ++g) {
             const LdsWindows<NW> wr(re, ub[g]), wi(im, ub[g]);
@@ -259,7 +360,7 @@ __device__ __forceinline__ void lds_write(const float2* v, int t, float* re, flo
                 if (SP::group(b) != g) continue;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int c = pad_any(SP::C(b, r));
+                    const int c = LY::pad(SP::C(b, r));
                     if constexpr (COMP & 1) wr.at(c) = v[b * R + r].x;
                     if constexpr (COMP & 2) wi.at(c) = v[b * R + r].y;
                 }
@@ -270,7 +371,7 @@ __device__ __forceinline__ void lds_write(const float2* v, int t, float* re, flo
         for (int b = 0; b < B; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int i = lds_pad(out_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+                const int i = ExchangeLayout<PL, S>::pad(out_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
                 if constexpr (COMP & 1) re[i] = v[b * R + r].x;
                 if constexpr (COMP & 2) im[i] = v[b * R + r].y;
             }
@@ -283,8 +384,9 @@ __device__ __forceinline__ void lds_read(float2* v, int t, const float* re, cons
     using SP = IndexSplit<PL, S, MIRROR, false>;
     constexpr int R = SP::R, B = SP::B;
     if constexpr (kIndexSplits<PL, S, MIRROR, false>) {
-        constexpr int NW = lds_span(PL::N) / kLdsWindow + 1;
-        const int ub[3] = {pad_any(SP::U(t, 0)), MIRROR ? pad_any(SP::U(t, 1)) : 0, MIRROR ? pad_any(SP::U(t, 2)) : 0};
+        constexpr int NW = plan_span<PL>() / kLdsWindow + 1;
+        using LY = typename SP::LY;
+        const int ub[3] = {LY::pad(SP::U(t, 0)), MIRROR ? LY::pad(SP::U(t, 1)) : 0, MIRROR ? LY::pad(SP::U(t, 2)) : 0};
 #pragma unroll
         for (int g = 0; g < (MIRROR ? 3 : 1); ++g) {
             const LdsWindows<NW> wr(const_cast<float*>(re), ub[g]), wi(const_cast<float*>(im), ub[g]);
@@ -293,7 +395,7 @@ __device__ __forceinline__ void lds_read(float2* v, int t, const float* re, cons
                 if (SP::group(b) != g) continue;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int c = pad_any(SP::C(b, r));
+                    const int c = LY::pad(SP::C(b, r));
                     if constexpr (COMP & 1) v[b * R + r].x = wr.at(c);
                     if constexpr (COMP & 2) v[b * R + r].y = wi.at(c);
                 }
@@ -304,7 +406,7 @@ __device__ __forceinline__ void lds_read(float2* v, int t, const float* re, cons
         for (int b = 0; b < B; ++b)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int i = lds_pad(in_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
+                const int i = ExchangeLayout<PL, S - 1>::pad(in_index<PL, S>(bfly<PL, S, MIRROR>(t, b), r));
                 if constexpr (COMP & 1) v[b * R + r].x = re[i];
                 if constexpr (COMP & 2) v[b * R + r].y = im[i];
             }
