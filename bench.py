@@ -105,7 +105,18 @@ def cpu_baseline(n: int, seconds: float = 10.0):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
+    # one core: one signal's forward + inverse at a time, median of 200 repetitions
+    single = None
     if use_ref:
+        reps = []
+        for k in range(200):
+            t1 = time.perf_counter()
+            lib.CkFftRealForward(ctx, n, x[k % sample].ctypes.data, yb[0].ctypes.data)
+            lib.CkFftRealInverse(ctx, n, yb[0].ctypes.data, zb[0].ctypes.data, tmps[0].ctypes.data)
+            reps.append(time.perf_counter() - t1)
+        med = float(np.median(reps))
+        single = {"value": round(2 * real_flops(n) / med / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                  "us_per_forward_plus_inverse": round(med * 1e6, 1), "sample": "median of 200 repetitions"}
         R.shutdown(ctx)
     gflops = passes * sample * 2 * real_flops(n) / el / 1e9
     cpu = platform.processor() or platform.machine()
@@ -121,7 +132,7 @@ def cpu_baseline(n: int, seconds: float = 10.0):
             "kind": "reference" if use_ref else "port",
             "sample": f"{passes} passes x {sample} signals x (real fwd + real inv), n={n}, "
                       f"{el:.1f} s wall, {threads} threads sharing one context",
-            "cpu": cpu}
+            "cpu": cpu, "single_core": single}
 
 
 def uneven_mix(torch, dev, count: int, seed: int):
@@ -510,6 +521,23 @@ def main():
     fwd_ms = back_to_back(lambda: ctx.real_forward_batch(n, B, x, y, stream))
     inv_ms = back_to_back(lambda: ctx.real_inverse_batch(n, B, y, z, stream))
 
+    # distribution: 100 launches of each kernel, each between its own pair of events (the
+    # events add a small gap to every launch), median and p99
+    def distribution(launch, reps=100):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        torch.cuda.synchronize()
+        for a, b in ev:
+            a.record(stream)
+            launch()
+            b.record(stream)
+        torch.cuda.synchronize()
+        t = np.array([a.elapsed_time(b) for a, b in ev])
+        return {"reps": reps, "median_ms": round(float(np.median(t)), 5), "p99_ms": round(float(np.percentile(t, 99)), 5)}
+
+    dist = {"real_forward": distribution(lambda: ctx.real_forward_batch(n, B, x, y, stream)),
+            "real_inverse": distribution(lambda: ctx.real_inverse_batch(n, B, y, z, stream)),
+            "timing": "each launch between its own pair of HIP events on the launch stream"}
+
     flops_step = B * 2 * real_flops(n)
     value = flops_step / (ms_per_step * 1e-3) / 1e9
     bytes_launch = B * real_bytes(n)
@@ -528,6 +556,7 @@ def main():
         "step_GBs": round(2 * bytes_launch / (ms_per_step * 1e-3) / 1e9, 1),
         "frac_per_launch": {"real_forward": round(bytes_launch / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "real_inverse": round(bytes_launch / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "launch_distribution": dist,
     }
 
     # parity of the benchmarked outputs (sample of signals) against the CPU oracle and FP64
