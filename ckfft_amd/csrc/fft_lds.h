@@ -380,7 +380,9 @@ struct LdsShape {
     static_assert(TPW >= 1 && TPW * T == WG, "workgroup must hold whole transforms");
     static_assert(!SPLIT || TPW == 1, "plane-by-plane exchange is for one transform per workgroup");
     static constexpr int SPAN = plan_span<PL>();
-    static constexpr int LDS_WORDS = T > 1 ? (SPLIT ? SPAN : 2 * SPAN * TPW) : 1;
+    // LDS holds the exchanges (any plan of more than one stage, which includes every plan
+    // of more than one thread)
+    static constexpr int LDS_WORDS = PL::NSTAGES > 1 ? (SPLIT ? SPAN : 2 * SPAN * TPW) : 1;
     static constexpr int MS = mirror_stage<PL, KIND>();
     // complex elements per signal on each side
     static constexpr int IN_STRIDE = KIND == kRealInverse ? N + 1 : N;
