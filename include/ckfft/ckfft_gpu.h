@@ -75,7 +75,8 @@ typedef struct {
 
 typedef struct _CkFftBatchPlan CkFftBatchPlan;
 
-/* Flag: capture the persistent launch in a hipGraph once and replay it. */
+/* Flag: capture the plan's launches (the persistent kernel and the table-driven passes)
+ * in a hipGraph once and replay it. */
 #define kCkFftBatch_Graph 1
 
 /* Validate the descriptors (same rules as the single-signal calls, plus device
