@@ -11,7 +11,12 @@
 //    DMA of chunk c);
 //  * the output comes back by DMA chunk by chunk into the pinned buffer and is copied out
 //    behind it (chunk c's CPU copy overlaps the DMA of chunk c+1);
-//  * pinned (hipHostMalloc) caller buffers are DMA'd directly, device buffers used in place.
+//  * pinned (hipHostMalloc) caller buffers are DMA'd directly, device buffers used in place;
+//  * a transform that reads its input once and writes its output once (one workgroup per
+//    signal: complex n <= 2^15, real n <= 2^16, FP32 mode) skips the DMA: the kernel reads
+//    and writes host memory itself (pinned caller buffers in place, pageable ones through
+//    the slot's pinned buffer).  A DMA costs a fixed ~10 us per copy on this path, more
+//    than moving these sizes over the bus.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -144,12 +149,38 @@ hipError_t grow(Slot* s, size_t need)
     return e;
 }
 
+// One pass over the data: the kernel may read and write host memory in place.
+bool one_pass(Kind kind, int log2n, const DeviceTables& tb)
+{
+    const bool real = kind == kRealForward || kind == kRealInverse;
+    return tb.precision == kCkFftPrecision_Fp32 && log2n <= (real ? kLogMaxLds + 1 : kLogMaxLds);
+}
+
+hipError_t run_direct(Slot* s, Kind kind, int log2n, const void* in, size_t ib, MemKind kin, void* out, size_t ob,
+                      MemKind kout, const DeviceTables& tb)
+{
+    const size_t ib_al = (ib + 255) & ~size_t(255);
+    const void* src = in;
+    void* dst = out;
+    if (kin == kHostPageable) {
+        std::memcpy(s->h_buf, in, ib);
+        src = s->h_buf;
+    }
+    if (kout == kHostPageable) dst = s->h_buf + ib_al;
+    hipError_t e = launch_transform(kind, log2n, 1, src, dst, tb, s->stream);
+    const hipError_t es = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = es;
+    if (e == hipSuccess && kout == kHostPageable) std::memcpy(out, dst, ob);
+    return e;
+}
+
 hipError_t run_staged(Slot* s, Kind kind, int log2n, const void* in, size_t ib, MemKind kin, void* out, size_t ob,
                       MemKind kout, const DeviceTables& tb)
 {
     const size_t ib_al = (ib + 255) & ~size_t(255);
     hipError_t e = grow(s, ib_al + ob);
     if (e != hipSuccess) return e;
+    if (one_pass(kind, log2n, tb)) return run_direct(s, kind, log2n, in, ib, kin, out, ob, kout, tb);
     char* d_in = s->d_buf;
     char* d_out = s->d_buf + ib_al;
     // input
