@@ -91,18 +91,13 @@ int fail(hipError_t e, const char* where)
     return 0;
 }
 
-DeviceTables tables(const CkFftContext* c)
-{
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw};
-}
-
 // One synchronous transform, as every drop-in call is: device buffers in place on the
 // null stream, host buffers through a staging slot.
 int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
 {
     const int log2n = ilog2(n);
     if (log2n > max_log2n(kind)) return 0;
-    return record(staged_transform(c->staging, kind, log2n, in, in_bytes(kind, n), out, out_bytes(kind, n), tables(c)),
+    return record(staged_transform(c->staging, kind, log2n, in, in_bytes(kind, n), out, out_bytes(kind, n), context_tables(c)),
                   "transform")
                ? 1
                : 0;
@@ -116,7 +111,7 @@ int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void
     if (log2n > max_log2n(kind)) return 0;
     if (count == 0) return 1;
     if (!is_device_pointer(in) || !is_device_pointer(out)) return 0;
-    hipError_t e = launch_transform(kind, log2n, count, in, out, tables(c), (hipStream_t)stream);
+    hipError_t e = launch_transform(kind, log2n, count, in, out, context_tables(c), (hipStream_t)stream);
     return record(e, "launch") ? 1 : 0;
 }
 
@@ -199,6 +194,20 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
         return nullptr;
     }
     StagingPool* pool = staging_create();
+    hipMemPool_t scratch = nullptr;
+    if (pool && !arena) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        uint64_t keep = UINT64_MAX;
+        if (!record(hipMemPoolCreate(&scratch, &props), "hipMemPoolCreate") ||
+            !record(hipMemPoolSetAttribute(scratch, hipMemPoolAttrReleaseThreshold, &keep), "pool threshold")) {
+            if (scratch) (void)hipMemPoolDestroy(scratch);
+            staging_destroy(pool);
+            pool = nullptr;
+        }
+    }
     if (!pool) {
         release();
         if (!buf) std::free(mem);
@@ -221,6 +230,7 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
     std::memset(c->tune, 0, sizeof(c->tune));
     c->precision = kCkFftPrecision_Fp32;
     c->staging = pool;
+    c->pool = scratch;
     return c;
 }
 
@@ -289,6 +299,8 @@ void CkFftShutdown(CkFftContext* context)
     if (!context || context->magic != kContextMagic) return;
     if (context->own_device) (void)hipFree(context->d_tables);
     staging_destroy(context->staging);
+    // returns at once; memory still held by unfinished frees is released when they finish
+    if (context->pool) (void)hipMemPoolDestroy(context->pool);
     const int own = context->own_host;
     context->magic = 0;
     context->~CkFftContext();

@@ -52,7 +52,12 @@ struct DeviceTables {
     int precision;           // kCkFftPrecision_* of the context
     const float2* xtw;       // transposed inter-pass twiddles (null when log_xtw == 0)
     int log_xtw;
+    hipMemPool_t pool;       // host-side handle: the pool of multi-pass intermediates (null:
+                             // the device's default pool)
 };
+
+// The tables (and pool) of a context, as the launchers take them.
+DeviceTables context_tables(const CkFftContext* c);
 
 // Launchers (return hipSuccess or an error).
 hipError_t launch_twiddle_pyramid(float2* table, int log_lmax, hipStream_t stream);
@@ -112,6 +117,10 @@ struct _CkFftContext {
     int precision;
     // Staging slots for host-pointer calls.
     ckfft_gpu::StagingPool* staging;
+    // Stream-ordered pool of the multi-pass intermediates; it keeps freed memory for reuse
+    // until CkFftShutdown.  Null for a context in a caller's device arena, whose
+    // intermediates are allocated and released per call.
+    hipMemPool_t pool;
 };
 
 constexpr uint64_t kContextMagic = 0x434b46465447505bull;  // "CKFFTGP["

@@ -80,7 +80,6 @@ hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const Ba
 // The same launch with the queue of `b` passed by value; requires at most kInlineQueue
 // items and transforms.
 hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, const DeviceTables& tb, hipStream_t s);
-DeviceTables context_tables(const CkFftContext* c);
 long long transform_bytes(int kind, int log2n);
 bool is_device_memory(const void* p);
 bool batch_error(hipError_t e, const char* where);

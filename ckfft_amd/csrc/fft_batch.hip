@@ -356,7 +356,8 @@ hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const Ba
 
 DeviceTables context_tables(const CkFftContext* c)
 {
-    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw};
+    return DeviceTables{c->d_pyramid, c->log_pyramid, c->d_fine, c->log_max, c->precision, c->d_xtw, c->log_xtw,
+                        c->pool};
 }
 
 }  // namespace ckfft_gpu

@@ -180,7 +180,9 @@ hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, voi
     const long long pts = (1ll << logc) * (long long)count;
     float2* tmp = nullptr;
     // real kinds need one more signal-sized buffer for the packed/unpacked half spectrum
-    hipError_t e = hipMallocAsync((void**)&tmp, sizeof(float2) * pts * (real ? 2 : 1), s);
+    const size_t tmp_bytes = sizeof(float2) * pts * (real ? 2 : 1);
+    hipError_t e = tb.pool ? hipMallocFromPoolAsync((void**)&tmp, tmp_bytes, tb.pool, s)
+                           : hipMallocAsync((void**)&tmp, tmp_bytes, s);
     if (e != hipSuccess) return e;
     const int threads = 256;
     if (kind == kComplexForward || kind == kComplexInverse) {
