@@ -113,14 +113,26 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
                     v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
                 }
         } else {
+            // The thread's points are r = tA + b*T + rr*M0 (M0 = R/R0), so
+            // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: B0 + log2(R0) table reads per
+            // thread instead of one per point.  Above 2^16 every read gathers from a fine
+            // table (and multiplies by a coarse entry); gathering one per point made the
+            // pass bound by the texture-address unit.
+            constexpr int M0 = R / R0;
+            TwBase<R0> sb;
+            sb.w[0] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)M0 * jm));
+            if constexpr (R0 >= 4) sb.w[1] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(2 * M0) * jm));
+            if constexpr (R0 >= 8) sb.w[2] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(4 * M0) * jm));
+            if constexpr (R0 >= 16) sb.w[3] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(8 * M0) * jm));
+            float2 step[R0];
+            twiddle_powers<R0>(step, sb);
 #pragma unroll
-            for (int b = 0; b < B0; ++b)
+            for (int b = 0; b < B0; ++b) {
+                const float2 wb = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(tA + b * T) * jm));
+                v[b * R0] = cmul(v[b * R0], wb);
 #pragma unroll
-                for (int rr = 0; rr < R0; ++rr) {
-                    const int r = in_index<PL, 0>(tA + b * T, rr);
-                    v[b * R0 + rr] =
-                        cmul(v[b * R0 + rr], dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)r * jm)));
-                }
+                for (int rr = 1; rr < R0; ++rr) v[b * R0 + rr] = cmul(v[b * R0 + rr], cmul(wb, step[rr]));
+            }
         }
     }
     stage_compute<PL, 0, INV>(v, tA, pyr);

@@ -152,6 +152,35 @@ __device__ __forceinline__ void twiddle_apply(float2* a, const TwBase<R>& tb)
     }
 }
 
+// The powers w^0 .. w^(R-1) from the power-of-two ones in tb, formed as twiddle_apply
+// forms them (w^0 = 1 exactly).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2* p, const TwBase<R>& tb)
+{
+    p[0] = make_float2(1.0f, 0.0f);
+    if constexpr (R >= 2) p[1] = tb.w[0];
+    if constexpr (R >= 4) {
+        p[2] = tb.w[1];
+        p[3] = cmul(p[1], p[2]);
+    }
+    if constexpr (R >= 8) {
+        p[4] = tb.w[2];
+        p[5] = cmul(p[1], p[4]);
+        p[6] = cmul(p[2], p[4]);
+        p[7] = cmul(p[3], p[4]);
+    }
+    if constexpr (R == 16) {
+        p[8] = tb.w[3];
+        p[9] = cmul(p[1], p[8]);
+        p[10] = cmul(p[2], p[8]);
+        p[11] = cmul(p[3], p[8]);
+        p[12] = cmul(p[4], p[8]);
+        p[13] = cmul(p[1], p[12]);
+        p[14] = cmul(p[2], p[12]);
+        p[15] = cmul(p[3], p[12]);
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void twiddle_mul(float2* a, const float2* __restrict__ lvl, int k)
 {
