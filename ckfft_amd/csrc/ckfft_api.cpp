@@ -194,6 +194,8 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
         return nullptr;
     }
     StagingPool* pool = staging_create();
+    // the intermediates' pool is an optimisation: without one (creation refused), they come
+    // from the device's default pool
     hipMemPool_t scratch = nullptr;
     if (pool && !arena) {
         hipMemPoolProps props{};
@@ -201,11 +203,13 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
         props.location.type = hipMemLocationTypeDevice;
         props.location.id = dev;
         uint64_t keep = UINT64_MAX;
-        if (!record(hipMemPoolCreate(&scratch, &props), "hipMemPoolCreate") ||
-            !record(hipMemPoolSetAttribute(scratch, hipMemPoolAttrReleaseThreshold, &keep), "pool threshold")) {
-            if (scratch) (void)hipMemPoolDestroy(scratch);
-            staging_destroy(pool);
-            pool = nullptr;
+        if (hipMemPoolCreate(&scratch, &props) != hipSuccess) {
+            (void)hipGetLastError();
+            scratch = nullptr;
+        } else if (hipMemPoolSetAttribute(scratch, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipMemPoolDestroy(scratch);
+            scratch = nullptr;
         }
     }
     if (!pool) {
