@@ -61,7 +61,7 @@ bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int inde
 // Where complex 2^16 transforms run.  Both compute the same two radix-2^8 passes:
 //  kExtra  an ordinary multi-pass launch each (the lower latency for a lone transform,
 //          used by the server);
-//  kTable  two launches per direction over all of them, tiles spread over the whole GPU
+//  kTable  two launches over all of them (both directions), tiles spread over the whole GPU
 //          (the higher throughput for many, used by batch plans).
 enum class TwoPass { kExtra, kTable };
 
@@ -71,9 +71,9 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
 // Scratch of the table-pass transforms: (1 << kPassLogN) points each.
 size_t batch_scratch_transforms(const BatchBuild& b);
 void batch_assign_scratch(BatchBuild& b, float2* scratch);
-// The table pass launches: pass 1 then pass 2 over `count` transforms of d_trs.
-hipError_t launch_pass_table(const BatchTransform* d_trs, int count, bool inverse, const DeviceTables& tb,
-                             hipStream_t s);
+// The table pass launches: pass 1 then pass 2 over the n_fwd forward transforms of d_trs
+// followed by its n_inv inverse ones.
+hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, const DeviceTables& tb, hipStream_t s);
 int persistent_grid(const CkFftContext* ctx);
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
                                  int grid, const DeviceTables& tb, hipStream_t s);

@@ -12,7 +12,7 @@
 //  * the last workgroup to leave resets the queue, so a plan re-executes (eagerly or as
 //    a replayed hipGraph) with no other launch.
 // Complex 2^16 transforms (above one workgroup's LDS) run after it as two table-driven
-// pass launches per direction over all of them, with the tiles of the stand-alone
+// pass launches over all of them (both directions), with the tiles of the stand-alone
 // multi-pass kernels; transforms neither takes (closed-form sizes, sizes above 2^16) run
 // through the ordinary launches, on the same stream.
 //
@@ -287,13 +287,15 @@ void batch_assign_scratch(BatchBuild& b, float2* scratch)
     for (auto& t : b.pass_inv) t.scratch = scratch + (k++ << kPassLogN);
 }
 
-// Complex 2^16 transforms listed in a table, each with its own input, output and
-// intermediate: one pass over all of them, tiles dealt over the whole GPU.  The same
-// tiles (and so the same results) as the stand-alone multi-pass launches.
-template <bool INV, bool FIRST>
+// Complex 2^16 transforms listed in a table (forward ones first, then inverse), each with
+// its own input, output and intermediate: one pass over all of them, both directions in
+// one launch, tiles dealt over the whole GPU.  The same tiles (and so the same results)
+// as the stand-alone multi-pass launches.
+template <bool FIRST>
 __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kernel(
-    const BatchTransform* __restrict__ trs, const float2* __restrict__ pyr, const float2* __restrict__ fine,
-    const float2* __restrict__ xtw, int log_xtw, long long tiles_per_signal, long long total_tiles)
+    const BatchTransform* __restrict__ trs, int n_fwd, const float2* __restrict__ pyr,
+    const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, long long tiles_per_signal,
+    long long total_tiles)
 {
     using SH = PassShape<kPassLogR>;
     __shared__ float lds[SH::CB * SH::RSTRIDE];
@@ -301,28 +303,32 @@ __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kerne
     const long long g = tile / tiles_per_signal;
     const long long q = tile - g * tiles_per_signal;
     const BatchTransform d = trs[g];
-    pass_tile<kPassLogR, SH::LOGCB, INV, FIRST>(FIRST ? d.in : d.scratch, FIRST ? d.scratch : d.out, q, kPassLogN,
-                                                FIRST ? 0 : kPassLogN - kPassLogR, pyr, fine, xtw, log_xtw, lds,
-                                                threadIdx.x);
+    const float2* src = FIRST ? d.in : d.scratch;
+    float2* dst = FIRST ? d.scratch : d.out;
+    constexpr int logns = FIRST ? 0 : kPassLogN - kPassLogR;
+    if (g < n_fwd)   // uniform over the workgroup
+        pass_tile<kPassLogR, SH::LOGCB, false, FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw, lds,
+                                                      threadIdx.x);
+    else
+        pass_tile<kPassLogR, SH::LOGCB, true, FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw, lds,
+                                                     threadIdx.x);
 }
 
-hipError_t launch_pass_table(const BatchTransform* d_trs, int count, bool inverse, const DeviceTables& tb,
-                             hipStream_t s)
+hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, const DeviceTables& tb, hipStream_t s)
 {
     static_assert(2 * kPassLogR == kPassLogN, "two equal passes");
+    const long long count = (long long)n_fwd + n_inv;
     if (count <= 0) return hipSuccess;
     using SH = PassShape<kPassLogR>;
     const long long tiles = (1ll << (kPassLogN - kPassLogR)) / SH::CB;
     const long long total = tiles * count;
     if (total > 0x7fffffffll) return hipErrorInvalidValue;
-    auto k1 = inverse ? fft_pass_table_kernel<true, true> : fft_pass_table_kernel<false, true>;
-    auto k2 = inverse ? fft_pass_table_kernel<true, false> : fft_pass_table_kernel<false, false>;
-    hipLaunchKernelGGL(k1, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, tb.pyramid, tb.fine, tb.xtw, tb.log_xtw,
-                       tiles, total);
+    hipLaunchKernelGGL(fft_pass_table_kernel<true>, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd,
+                       tb.pyramid, tb.fine, tb.xtw, tb.log_xtw, tiles, total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k2, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, tb.pyramid, tb.fine, tb.xtw, tb.log_xtw,
-                       tiles, total);
+    hipLaunchKernelGGL(fft_pass_table_kernel<false>, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd,
+                       tb.pyramid, tb.fine, tb.xtw, tb.log_xtw, tiles, total);
     return hipGetLastError();
 }
 
@@ -444,8 +450,7 @@ hipError_t launch_plan_kernels(CkFftBatchPlan* p, hipStream_t s)
     const DeviceTables tb = context_tables(p->ctx);
     hipError_t e = hipSuccess;
     if (p->n_items) e = launch_persistent_raw(p->d_items, p->n_items, p->d_trs, p->d_ctl, p->grid, tb, s);
-    if (e == hipSuccess) e = launch_pass_table(p->d_pass, p->n_pass_fwd, false, tb, s);
-    if (e == hipSuccess) e = launch_pass_table(p->d_pass + p->n_pass_fwd, p->n_pass_inv, true, tb, s);
+    if (e == hipSuccess) e = launch_pass_table(p->d_pass, p->n_pass_fwd, p->n_pass_inv, tb, s);
     return e;
 }
 

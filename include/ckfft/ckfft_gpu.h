@@ -40,8 +40,8 @@ int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
  * A batch plan takes any mix of transforms -- each with its own kind, size and device
  * buffers -- and runs them as one launch of a persistent kernel (workgroups pull work
  * items from an on-device queue, largest first).  Complex 2^16 transforms run as two
- * table-driven pass launches per direction over all of them (every workgroup of the GPU
- * on one transform's tiles at a time), and transforms the persistent kernel does not take
+ * table-driven pass launches over all of them, both directions (every workgroup of the
+ * GPU on one transform's tiles at a time), and transforms the persistent kernel does not take
  * (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches; all on the same
  * stream after the persistent launch.  CkFftBatchInfo reports how many transforms each
  * route takes.
