@@ -65,22 +65,32 @@ __global__ __launch_bounds__(256) void real_pack_kernel(const float2* __restrict
     if (k != h - k) o[h - k] = make_float2(a.x - c.y, -(a.y + c.x));
 }
 
-// Real inverse: Y (h+1 per signal) -> t (h complex points per signal), one point per thread.
-__global__ __launch_bounds__(256) void real_unpack_kernel(const float2* __restrict__ y, float2* __restrict__ t,
-                                                          const float2* __restrict__ pyr,
-                                                          const float2* __restrict__ fine, int logn, long long count)
+// Real inverse: Y (h+1 per signal) -> t (h complex points per signal).  Thread k unpacks
+// the pair (k, h-k), k in [0, h/2], reading each Y value once; each point is formed from
+// its own value, its mirror and its own twiddle.
+__device__ __forceinline__ float2 unpack_point(float2 yk, float2 ym, float2 w)
 {
-    const long long h = 1ll << (logn - 1);
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= h * count) return;
-    const long long g = e >> (logn - 1);
-    const long long k = e & (h - 1);
-    const float2* ys = y + g * (h + 1);
-    const float2 yk = ys[k], ym = ys[h - k];
     const float2 a = make_float2(yk.x + ym.x, yk.y - ym.y);
     const float2 d = make_float2(yk.x - ym.x, yk.y + ym.y);
-    const float2 c = cmul(cconj(twiddle_at(pyr, fine, logn, k)), d);
-    t[g * h + k] = make_float2(a.x - c.y, a.y + c.x);
+    const float2 c = cmul(cconj(w), d);
+    return make_float2(a.x - c.y, a.y + c.x);
+}
+
+__global__ __launch_bounds__(256) void real_unpack_kernel(const float2* __restrict__ y, float2* __restrict__ t,
+                                                          const float2* __restrict__ pyr,
+                                                          const float2* __restrict__ fine, int logn, long long pairs,
+                                                          long long count)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pairs * count) return;
+    const long long g = e / pairs;
+    const long long k = e - g * pairs;
+    const long long h = 1ll << (logn - 1);
+    const float2* ys = y + g * (h + 1);
+    float2* ts = t + g * h;
+    const float2 yk = ys[k], ym = ys[h - k];
+    ts[k] = unpack_point(yk, ym, twiddle_at(pyr, fine, logn, k));
+    if (k != 0 && k != h - k) ts[h - k] = unpack_point(ym, yk, twiddle_at(pyr, fine, logn, h - k));
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -199,8 +209,10 @@ hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, voi
         }
     } else {
         float2* t = tmp + pts;
-        hipLaunchKernelGGL(real_unpack_kernel, dim3((unsigned)((pts + threads - 1) / threads)), dim3(threads), 0, s,
-                           (const float2*)in, t, tb.pyramid, tb.fine, log2n, (long long)count);
+        const long long pairs = (1ll << (logc - 1)) + 1;    // k = 0 .. h/2
+        const long long n_thr = pairs * count;
+        hipLaunchKernelGGL(real_unpack_kernel, dim3((unsigned)((n_thr + threads - 1) / threads)), dim3(threads), 0, s,
+                           (const float2*)in, t, tb.pyramid, tb.fine, log2n, pairs, (long long)count);
         e = hipGetLastError();
         if (e == hipSuccess) e = run_passes(true, logc, count, t, (float2*)out, tmp, tb, s);
     }
