@@ -62,12 +62,59 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
     }
 }
 
+// Inter-pass twiddles of butterfly column jm in a pass of span 2^logns: the thread's point r
+// (r = in_index of its first-stage element) is multiplied by W_L^(r jm), L = 2^logns * R.
+// XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
+// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time.
+template <class PL, bool INV, int XTW>
+__device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, int logns,
+                                              const float2* __restrict__ pyr, const float2* __restrict__ fine,
+                                              const float2* __restrict__ xtw, int log_xtw)
+{
+    constexpr int LOGR = PL::nslog(PL::NSTAGES);              // log2 of the pass radix
+    constexpr int R = PL::N, T = PL::T;
+    constexpr int R0 = 1 << PL::rlog(0);
+    constexpr int B0 = PL::P / R0;
+    const int loglev = logns + LOGR;
+    if (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw) {
+        // the same values from the transposed copy: neighbouring columns, neighbouring words
+        const float2* tcol = xtw + jm;
+        const int sh = log_xtw - loglev;
+#pragma unroll
+        for (int b = 0; b < B0; ++b)
+#pragma unroll
+            for (int rr = 0; rr < R0; ++rr) {
+                const int r = in_index<PL, 0>(tA + b * T, rr);
+                v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
+            }
+    } else {
+        // The thread's points are r = tA + b*T + rr*M0 (M0 = R/R0), so
+        // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: B0 + log2(R0) table reads per
+        // thread instead of one per point.  Above 2^16 every read gathers from a fine
+        // table (and multiplies by a coarse entry); gathering one per point made the
+        // pass bound by the texture-address unit.
+        constexpr int M0 = R / R0;
+        TwBase<R0> sb;
+        sb.w[0] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)M0 * jm));
+        if constexpr (R0 >= 4) sb.w[1] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(2 * M0) * jm));
+        if constexpr (R0 >= 8) sb.w[2] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(4 * M0) * jm));
+        if constexpr (R0 >= 16) sb.w[3] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(8 * M0) * jm));
+        float2 step[R0];
+        twiddle_powers<R0>(step, sb);
+#pragma unroll
+        for (int b = 0; b < B0; ++b) {
+            const float2 wb = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(tA + b * T) * jm));
+            v[b * R0] = cmul(v[b * R0], wb);
+#pragma unroll
+            for (int rr = 1; rr < R0; ++rr) v[b * R0 + rr] = cmul(v[b * R0 + rr], cmul(wb, step[rr]));
+        }
+    }
+}
+
 // One tile of one pass: butterflies q*CB .. q*CB + CB-1 of the signal at `src` (2^logn
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
 // `lds` holds CB * RSTRIDE floats.  Used by the pass kernels here and by the batch
-// plans' table-driven passes.
-// XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
-// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time.
+// plans' table-driven passes.  XTW as in pass_twiddles.
 template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
 __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
                                           int logn, int logns, const float2* __restrict__ pyr,
@@ -98,43 +145,8 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
             const int r = in_index<PL, 0>(tA + b * T, rr);
             v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
         }
-    if constexpr (!FIRST) {
-        const long long jm = (j0 + cA) & ((1ll << logns) - 1);
-        const int loglev = logns + LOGR;
-        if (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw) {
-            // the same values from the transposed copy: neighbouring columns, neighbouring words
-            const float2* tcol = xtw + jm;
-            const int sh = log_xtw - loglev;
-#pragma unroll
-            for (int b = 0; b < B0; ++b)
-#pragma unroll
-                for (int rr = 0; rr < R0; ++rr) {
-                    const int r = in_index<PL, 0>(tA + b * T, rr);
-                    v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
-                }
-        } else {
-            // The thread's points are r = tA + b*T + rr*M0 (M0 = R/R0), so
-            // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: B0 + log2(R0) table reads per
-            // thread instead of one per point.  Above 2^16 every read gathers from a fine
-            // table (and multiplies by a coarse entry); gathering one per point made the
-            // pass bound by the texture-address unit.
-            constexpr int M0 = R / R0;
-            TwBase<R0> sb;
-            sb.w[0] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)M0 * jm));
-            if constexpr (R0 >= 4) sb.w[1] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(2 * M0) * jm));
-            if constexpr (R0 >= 8) sb.w[2] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(4 * M0) * jm));
-            if constexpr (R0 >= 16) sb.w[3] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(8 * M0) * jm));
-            float2 step[R0];
-            twiddle_powers<R0>(step, sb);
-#pragma unroll
-            for (int b = 0; b < B0; ++b) {
-                const float2 wb = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(tA + b * T) * jm));
-                v[b * R0] = cmul(v[b * R0], wb);
-#pragma unroll
-                for (int rr = 1; rr < R0; ++rr) v[b * R0 + rr] = cmul(v[b * R0 + rr], cmul(wb, step[rr]));
-            }
-        }
-    }
+    if constexpr (!FIRST)
+        pass_twiddles<PL, INV, XTW>(v, tA, (j0 + cA) & ((1ll << logns) - 1), logns, pyr, fine, xtw, log_xtw);
     stage_compute<PL, 0, INV>(v, tA, pyr);
     pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
 

@@ -326,7 +326,8 @@ static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_a
     const bool real = d.kind >= kRealForward;
     const bool multipass = real ? log2n > kLogMaxLds + 1 : log2n > kLogMaxLds;
     const bool mfma = S->ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n);
-    const long long bytes = transform_bytes(d.kind, log2n) + (multipass && !mfma ? 8ll << log2n : 0);
+    // (one half-size complex signal, 8 * n/2 bytes, for a real kind)
+    const long long bytes = transform_bytes(d.kind, log2n) + (multipass && !mfma ? (real ? 4ll : 8ll) << log2n : 0);
     if (bytes > S->budget) {
         std::snprintf(g_serve_error, sizeof(g_serve_error), "request larger than the admission budget");
         return -1;

@@ -106,8 +106,9 @@ int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
  * arriving while others run join the next batch and their work overlaps on the device.
  * Admission is first-come-first-served against an HBM budget: a request is admitted
  * while the bytes of admitted, unfinished requests (one read of the input, one write of
- * the output, plus 8n bytes of intermediate for the multi-pass sizes) plus the server's
- * own resident buffers stay within hbmBudget; others wait (back-pressure, not failure).
+ * the output, plus the intermediate of the multi-pass sizes: 8n bytes complex, 4n real)
+ * plus the server's own resident buffers stay within hbmBudget; others wait
+ * (back-pressure, not failure).
  * Twiddles, descriptor buffers and streams stay resident.
  * A request's result is bit-identical to the single-signal call on the same input,
  * whatever it was batched with.  Buffers are device memory owned by the caller and
