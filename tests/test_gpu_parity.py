@@ -437,8 +437,11 @@ def test_complex_multipass_sizes(torch, ctx_big, kind, k):
 
 
 @pytest.mark.parametrize("kind", ["rfwd", "rinv"])
-@pytest.mark.parametrize("k", [17, 18, 20, 22])
+@pytest.mark.parametrize("k", [17, 18, 19, 20, 21, 22, 23, 24, 25])
 def test_real_multipass_sizes(torch, ctx_big, kind, k):
+    """Half-size complex passes with the pack fused into the last pass (forward) or the
+    unpack into the first (inverse): 2 passes up to 2^23 real samples, 3 from 2^24 (where
+    an intermediate goes through the output buffer)."""
     n = 1 << k
     count = 3 if k <= 18 else 1
     x = make_input(kind, n, count, 9000 + k)
