@@ -23,7 +23,10 @@
 
 namespace ckfft_gpu {
 
-template <int LOGR, bool INV, bool FIRST>
+// XTW_PASS: the host has checked that the transposed twiddles cover this pass (span
+// 2^kLogXtwCols, level <= 2^log_xtw; possible for radix <= 2^8 only); otherwise the
+// twiddles come from the pyramid and fine tables.  One code path per kernel.
+template <int LOGR, bool INV, bool FIRST, bool XTW_PASS>
 __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ pyr,
     const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, int logn, int logns,
@@ -35,8 +38,8 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const long long g = tile / tiles_per_signal;
     const long long q = tile - g * tiles_per_signal;
     const long long N = 1ll << logn;
-    // the transposed twiddles cover span 2^kLogXtwCols at levels <= 2^16: radix <= 2^8
-    constexpr int XTW = LOGR + kLogXtwCols <= kLogMaxPyramid ? 1 : 0;
+    static_assert(!XTW_PASS || LOGR + kLogXtwCols <= kLogMaxPyramid, "transposed twiddles cover radix <= 2^8");
+    constexpr int XTW = XTW_PASS ? 2 : 0;
     pass_tile<LOGR, SH::LOGCB, INV, FIRST, XTW>(in + g * N, out + g * N, q, logn, logns, pyr, fine, xtw, log_xtw,
                                                 lds, threadIdx.x);
 }
@@ -154,8 +157,8 @@ __device__ __forceinline__ void pack_out(float2* __restrict__ o, long long k, lo
 }
 
 // Real forward: the last pass of the half-size transform (span ncols = h/R) writing the
-// packed spectrum (h+1 values per signal) instead of Z.
-template <int LOGR>
+// packed spectrum (h+1 values per signal) instead of Z.  XTW_PASS as in fft_pass_kernel.
+template <int LOGR, bool XTW_PASS>
 __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_pack_kernel(
     const float2* __restrict__ z, float2* __restrict__ out, const float2* __restrict__ pyr,
     const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, int logn,
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_pack_kernel(
     using PL = Plan<LOGR, SH::LOGP>;
     constexpr int R = SH::R, T = SH::T, CB = SH::CB, WG = SH::WG;
     constexpr int R0 = 1 << PL::rlog(0), B0 = PL::P / R0;
-    constexpr int XTW = LOGR + kLogXtwCols <= kLogMaxPyramid ? 1 : 0;
+    static_assert(!XTW_PASS || LOGR + kLogXtwCols <= kLogMaxPyramid, "transposed twiddles cover radix <= 2^8");
     __shared__ float lds[CB * SH::RSTRIDE];
     const long long tile = xcd_tile(blockIdx.x, total_tiles);
     const long long g = tile / tiles_per_signal;
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_pack_kernel(
     for (int b = 0; b < B0; ++b)
 #pragma unroll
         for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = zs[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols];
-    pass_twiddles<PL, false, XTW>(v, tA, col, logns, pyr, fine, xtw, log_xtw);
+    pass_twiddles<PL, false, XTW_PASS ? 2 : 0>(v, tA, col, logns, pyr, fine, xtw, log_xtw);
     float* reA = lds + cA * SH::RSTRIDE;
     float* imA = reA + SH::SPAN;
     float* reB = lds + cB * SH::RSTRIDE;
@@ -259,27 +262,33 @@ struct PassCfg {
     int cb;
 };
 
-template <int LOGR, bool INV, bool FIRST>
+template <int LOGR, bool INV, bool FIRST, bool XTW_PASS>
 constexpr PassCfg pass_cfg()
 {
-    return PassCfg{fft_pass_kernel<LOGR, INV, FIRST>, PassShape<LOGR>::WG, PassShape<LOGR>::CB};
+    // radices above 2^8 have no transposed-twiddle variant
+    constexpr bool X = XTW_PASS && LOGR + kLogXtwCols <= kLogMaxPyramid;
+    return PassCfg{fft_pass_kernel<LOGR, INV, FIRST, X>, PassShape<LOGR>::WG, PassShape<LOGR>::CB};
 }
 
-template <bool INV, bool FIRST, int... L>
+template <bool INV, bool FIRST, bool XTW_PASS, int... L>
 constexpr void fill_pass(PassCfg* tab, std::integer_sequence<int, L...>)
 {
-    ((tab[L + kLogMinPass] = pass_cfg<L + kLogMinPass, INV, FIRST>()), ...);
+    ((tab[L + kLogMinPass] = pass_cfg<L + kLogMinPass, INV, FIRST, XTW_PASS>()), ...);
 }
 
+// Pass kernels by [inverse][variant][log2 radix]; variant 0 = first pass, 1 = later pass,
+// 2 = later pass with twiddles from the transposed copy.
 struct PassTables {
-    PassCfg t[2][2][kLogMaxPass + 1];     // [inverse][first][log2 radix]
+    PassCfg t[2][3][kLogMaxPass + 1];
     PassTables()
     {
-        constexpr int NP = kLogMaxPass - kLogMinPass + 1;
-        fill_pass<false, false>(t[0][0], std::make_integer_sequence<int, NP>{});
-        fill_pass<false, true>(t[0][1], std::make_integer_sequence<int, NP>{});
-        fill_pass<true, false>(t[1][0], std::make_integer_sequence<int, NP>{});
-        fill_pass<true, true>(t[1][1], std::make_integer_sequence<int, NP>{});
+        constexpr auto L = std::make_integer_sequence<int, kLogMaxPass - kLogMinPass + 1>{};
+        fill_pass<false, true, false>(t[0][0], L);
+        fill_pass<false, false, false>(t[0][1], L);
+        fill_pass<false, false, true>(t[0][2], L);
+        fill_pass<true, true, false>(t[1][0], L);
+        fill_pass<true, false, false>(t[1][1], L);
+        fill_pass<true, false, true>(t[1][2], L);
     }
 };
 
@@ -296,17 +305,24 @@ using PackFn = void (*)(const float2*, float2*, const float2*, const float2*, co
 
 struct EdgePassCfg {
     UnpackFn unpack;
-    PackFn pack;
+    PackFn pack;       // twiddles from the pyramid / fine tables
+    PackFn pack_xtw;   // twiddles from the transposed copy
     int wg;
     int cb;
 };
 
+template <int LOGR>
+constexpr EdgePassCfg edge_cfg()
+{
+    constexpr bool X = LOGR + kLogXtwCols <= kLogMaxPyramid;
+    return EdgePassCfg{fft_pass_unpack_kernel<LOGR>, fft_pass_pack_kernel<LOGR, false>, fft_pass_pack_kernel<LOGR, X>,
+                       PassShape<LOGR>::WG, PassShape<LOGR>::CB};
+}
+
 template <int... L>
 constexpr void fill_edge(EdgePassCfg* tab, std::integer_sequence<int, L...>)
 {
-    ((tab[L + kLogMinPass] = EdgePassCfg{fft_pass_unpack_kernel<L + kLogMinPass>, fft_pass_pack_kernel<L + kLogMinPass>,
-                                         PassShape<L + kLogMinPass>::WG, PassShape<L + kLogMinPass>::CB}),
-     ...);
+    ((tab[L + kLogMinPass] = edge_cfg<L + kLogMinPass>()), ...);
 }
 
 struct EdgePassTables {
@@ -332,10 +348,18 @@ int plan_passes(int logc, int* logr)
     return np;
 }
 
+// The transposed twiddles cover a later pass of radix 2^logr and span 2^logns when the span
+// is their column count and the pass level 2^(logns+logr) is within their table.
+static bool xtw_covers(const DeviceTables& tb, int logr, int logns)
+{
+    return tb.xtw && logns == kLogXtwCols && logns + logr <= tb.log_xtw && logr + kLogXtwCols <= kLogMaxPyramid;
+}
+
 static hipError_t launch_pass(int logr, bool inv, bool first, const float2* in, float2* out, const DeviceTables& tb,
                               int logc, int logns, long long count, hipStream_t s)
 {
-    const PassCfg& c = pass_tables().t[inv][first][logr];
+    const bool xtw = !first && xtw_covers(tb, logr, logns);
+    const PassCfg& c = pass_tables().t[inv][first ? 0 : (xtw ? 2 : 1)][logr];
     const long long tiles = ((1ll << logc) >> logr) / c.cb;
     const long long total = tiles * count;
     if (total > 0x7fffffffll) return hipErrorInvalidValue;
@@ -375,7 +399,7 @@ static hipError_t run_passes(Kind edge, bool inv, int logc, long long count, con
                 hipLaunchKernelGGL(c.unpack, dim3((unsigned)total), dim3(c.wg), 0, s, src, dst, tb.pyramid, tb.fine,
                                    logn_real, tiles, total);
             else
-                hipLaunchKernelGGL(c.pack, dim3((unsigned)total), dim3(c.wg), 0, s, src, dst, tb.pyramid, tb.fine,
+                hipLaunchKernelGGL(xtw_covers(tb, logr[p], logns) ? c.pack_xtw : c.pack, dim3((unsigned)total), dim3(c.wg), 0, s, src, dst, tb.pyramid, tb.fine,
                                    tb.xtw, tb.log_xtw, logn_real, tiles, total);
             e = hipGetLastError();
         } else {
