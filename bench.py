@@ -8,7 +8,7 @@ events on the launch stream, which gives the roofline fraction against 8 TB/s.
 
 Also reported on the same line:
   * batched_uneven: 20,000 transforms of mixed sizes (2^6..2^16), kinds and directions,
-    each with its own buffers, as one persistent launch (autotuned, graph-replayed);
+    each with its own buffers, in two persistent launches (autotuned, graph-replayed);
   * plain: the same signals through the one-signal-per-call drop-in entry points
     (CkFftRealForward / CkFftRealInverse with device pointers, synchronous);
   * cpu_baseline: the reference's CPU path (oracle/_ref, built from the reference's
@@ -161,7 +161,7 @@ def uneven_mix(torch, dev, count: int, seed: int):
 
 
 def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 2000):
-    """20,000 transforms of the uneven mix run as one persistent launch (autotuned,
+    """20,000 transforms of the uneven mix run in two persistent launches (autotuned,
     replayed as a hipGraph).  FLOPs: 5 n log2 n complex, 2.5 n log2 n real."""
     count = 20000
     m = uneven_mix(torch, dev, count, 0xC4)
@@ -216,7 +216,7 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
         "workload": "20000 uneven transforms, n=2^k k~U{6..16}, 50% complex/real, 50% fwd/inv, own buffers",
         "ms_per_batch": round(ms, 4), "value": round(flops / (ms * 1e-3) / 1e9, 1), "unit": "GFLOP/s",
         "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-        "algorithmic_bytes": info["algorithmic_bytes"], "launch": "one persistent kernel + table-driven passes for complex 2^16, one hipGraph replay",
+        "algorithmic_bytes": info["algorithmic_bytes"], "launch": "wide and narrow persistent kernels + table-driven passes for complex 2^16, one hipGraph replay",
         "pass_transforms": info["pass_transforms"],
         "work_items": info["work_items"], "workgroups": info["workgroups"],
         "extra_launches": info["extra_launches"], "autotuned": tuned, "autotune_s": round(tune_s, 2),
@@ -614,7 +614,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(n)
     ctx.close()
     del x, y, z
-    uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=2)
+    uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=10)
     serving = None if args.no_serve else serving_leg(torch, C, dev)
     split = None if args.no_split else split_bf16_leg(torch, C, O, dev)
     configs = None

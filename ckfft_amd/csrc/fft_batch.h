@@ -45,7 +45,8 @@ struct InlineQueue {
 
 // Host description of a batch.
 struct BatchBuild {
-    std::vector<BatchItem> items;
+    std::vector<BatchItem> items;     // the wide kernel's queue, then the narrow kernel's
+    int n_wide = 0;                   // items of the wide kernel's queue
     std::vector<BatchTransform> trs;  // the persistent kernel's transforms, by class
     std::vector<BatchExtra> extras;
     // complex 2^16 transforms run by the table pass launches (TwoPass::kTable), forward
@@ -66,17 +67,22 @@ bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int inde
 enum class TwoPass { kExtra, kTable };
 
 // Validate and group a batch; var_override >= 0 forces one thread layout for every class.
+// narrow: classes the narrow persistent kernel takes (two workgroups per CU) go to its
+// queue; otherwise every item goes to the wide kernel (one workgroup per CU, any class).
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out, TwoPass two_pass);
+                 bool check_pointers, BatchBuild& out, TwoPass two_pass, bool narrow);
 // Scratch of the table-pass transforms: (1 << kPassLogN) points each.
 size_t batch_scratch_transforms(const BatchBuild& b);
 void batch_assign_scratch(BatchBuild& b, float2* scratch);
 // The table pass launches: pass 1 then pass 2 over the n_fwd forward transforms of d_trs
 // followed by its n_inv inverse ones.
 hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, const DeviceTables& tb, hipStream_t s);
-int persistent_grid(const CkFftContext* ctx);
+// Workgroups that fill the GPU with the wide (narrow) persistent kernel.
+int persistent_grid(const CkFftContext* ctx, bool narrow);
+// One launch of the wide (narrow) persistent kernel over n_items items; d_ctl holds its
+// kCtlWords control words, zero before the first launch (the kernel leaves them zero).
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
-                                 int grid, const DeviceTables& tb, hipStream_t s);
+                                 int grid, const DeviceTables& tb, hipStream_t s, bool narrow);
 // The same launch with the queue of `b` passed by value; requires at most kInlineQueue
 // items and transforms.
 hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, const DeviceTables& tb, hipStream_t s);

@@ -36,24 +36,40 @@
 
 namespace ckfft_gpu {
 
-constexpr int kPersistLdsFloats = 34816;        // 139264 bytes
 constexpr int kPassLogR = 8;                    // radix of both passes of a complex 2^16
 
+// LDS of the two persistent kernels: the wide one (139,264 B, one workgroup per CU) holds
+// every class; the narrow one (69,632 B, two workgroups per CU, so that one item's loads
+// overlap another's arithmetic) holds the classes for which narrow_class() is true.
+constexpr int kPersistLdsFloats = 34816;
+constexpr int kNarrowLdsFloats = 17408;
+
+// Points per thread (log2) of a class.  2^5 takes 16 in both variants: 32 points in one
+// thread would be a two-stage transform in a single thread, which lds_transform does not do.
 __host__ __device__ constexpr int persist_logp(int logc, int var)
 {
-    return logc <= 4 ? logc : logc <= 13 ? (var ? 5 : 4) : (logc == 14 ? 5 : 6);
+    return logc <= 4 ? logc : logc == 5 ? 4 : logc <= 13 ? (var ? 5 : 4) : (logc == 14 ? 5 : 6);
+}
+
+// The narrow kernel takes 16 points per thread up to 2^13 complex points (two planes of
+// 512 x 16 words) and 2^14 at 32 points per thread with one plane at a time.
+__host__ __device__ constexpr bool narrow_class(int logc, int var)
+{
+    return logc <= 4 || (logc <= 13 && persist_logp(logc, var) == 4) || logc == 14;
 }
 
 __host__ __device__ constexpr int lds_cls(int kind, int logc, int var) { return (kind * 16 + logc) * 2 + var; }
 
-template <int KIND, int LOGC, int VAR>
+template <bool NARROW, int KIND, int LOGC, int VAR>
 __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
                                                   const float2* __restrict__ tw, float* lds)
 {
     constexpr int LP = persist_logp(LOGC, VAR);
-    constexpr bool SPLIT = LOGC >= 15;
+    // one LDS plane at a time where both do not fit (the arithmetic is the same)
+    constexpr bool SPLIT = LOGC >= (NARROW ? 14 : 15);
     using SH = LdsShape<LOGC, LP, kPersistWG, KIND, SPLIT>;
-    static_assert(SH::LDS_WORDS <= kPersistLdsFloats, "class does not fit the persistent LDS buffer");
+    static_assert(SH::LDS_WORDS <= (NARROW ? kNarrowLdsFloats : kPersistLdsFloats),
+                  "class does not fit the persistent LDS buffer");
     // An opaque copy of the thread index: without it the compiler shares the (identical)
     // address arithmetic of the kernel's ~100 class bodies, hoists it above the dispatch
     // and keeps it live through every body, which spills the large classes.
@@ -66,33 +82,39 @@ __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTran
     lds_transform<LOGC, LP, kPersistWG, KIND, SPLIT>(d.in, d.out, active, tw, lds, tid);
 }
 
-// Classes whose two variants coincide (n <= 16 or n >= 2^14) are instantiated once.
-template <int KIND, int LOGC>
+// Classes whose two variants coincide (n <= 16 or n >= 2^14) are instantiated once; the
+// narrow kernel is only given the variant narrow_class() admits.
+template <bool NARROW, int KIND, int LOGC>
 __device__ __forceinline__ int run_lds_var(int var, const BatchItem& w, const BatchTransform* __restrict__ trs,
                                            const float2* __restrict__ tw, float* lds)
 {
     if constexpr (persist_logp(LOGC, 0) == persist_logp(LOGC, 1)) {
-        run_lds_item<KIND, LOGC, 0>(w, trs, tw, lds);
+        run_lds_item<NARROW, KIND, LOGC, 0>(w, trs, tw, lds);
+    } else if constexpr (NARROW) {
+        static_assert(narrow_class(LOGC, 0) && !narrow_class(LOGC, 1), "narrow variant");
+        run_lds_item<NARROW, KIND, LOGC, 0>(w, trs, tw, lds);
     } else {
         if (var)
-            run_lds_item<KIND, LOGC, 1>(w, trs, tw, lds);
+            run_lds_item<NARROW, KIND, LOGC, 1>(w, trs, tw, lds);
         else
-            run_lds_item<KIND, LOGC, 0>(w, trs, tw, lds);
+            run_lds_item<NARROW, KIND, LOGC, 0>(w, trs, tw, lds);
     }
     return 0;
 }
 
-template <int KIND, int LO, int... L>
+template <bool NARROW, int KIND, int LO, int... L>
 __device__ __forceinline__ void dispatch_lds(int logc, int var, const BatchItem& w,
                                              const BatchTransform* __restrict__ trs,
                                              const float2* __restrict__ tw, float* lds,
                                              std::integer_sequence<int, L...>)
 {
-    (void)((logc == LO + L ? run_lds_var<KIND, LO + L>(var, w, trs, tw, lds) : 0), ...);
+    (void)((logc == LO + L ? run_lds_var<NARROW, KIND, LO + L>(var, w, trs, tw, lds) : 0), ...);
 }
 
-// The persistent loop, shared by the kernel that reads its queue from device memory and
-// the one that receives a small queue by value in its arguments.
+// The persistent loop, shared by the kernels that read their queue from device memory and
+// the one that receives a small queue by value in its arguments.  NARROW: complex classes
+// up to 2^14 points (real up to 2^15), narrow variants only.
+template <bool NARROW>
 __device__ __forceinline__ void persistent_body(const BatchItem* __restrict__ items, int n_items,
                                                 const BatchTransform* __restrict__ trs, int* __restrict__ ctl,
                                                 const DeviceTables& tb, float* lds, int& s_item)
@@ -110,18 +132,24 @@ __device__ __forceinline__ void persistent_body(const BatchItem* __restrict__ it
         const int var = w.cls & 1;
         const int kind = (w.cls >> 1) >> 4;
         const int logc = (w.cls >> 1) & 15;
+        // complex log2 sizes 1..15 (real 2..15; the narrow kernel stops at 14)
+        constexpr int NC = NARROW ? 14 : 15, NR = NC - 1;
         switch (kind) {
         case kComplexForward:
-            dispatch_lds<kComplexForward, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+            dispatch_lds<NARROW, kComplexForward, 1>(logc, var, w, trs, tb.pyramid, lds,
+                                                     std::make_integer_sequence<int, NC>{});
             break;
         case kComplexInverse:
-            dispatch_lds<kComplexInverse, 1>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 15>{});
+            dispatch_lds<NARROW, kComplexInverse, 1>(logc, var, w, trs, tb.pyramid, lds,
+                                                     std::make_integer_sequence<int, NC>{});
             break;
         case kRealForward:
-            dispatch_lds<kRealForward, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+            dispatch_lds<NARROW, kRealForward, 2>(logc, var, w, trs, tb.pyramid, lds,
+                                                  std::make_integer_sequence<int, NR>{});
             break;
         default:
-            dispatch_lds<kRealInverse, 2>(logc, var, w, trs, tb.pyramid, lds, std::make_integer_sequence<int, 14>{});
+            dispatch_lds<NARROW, kRealInverse, 2>(logc, var, w, trs, tb.pyramid, lds,
+                                                  std::make_integer_sequence<int, NR>{});
             break;
         }
         __syncthreads();   // the LDS buffer is reused by the next item
@@ -143,16 +171,28 @@ __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchI
 {
     __shared__ float lds[kPersistLdsFloats];
     __shared__ int s_item;
-    persistent_body(items, n_items, trs, ctl, tb, lds, s_item);
+    persistent_body<false>(items, n_items, trs, ctl, tb, lds, s_item);
 }
 
-// Same, with the queue in the kernel arguments: a launch that needs no upload.
+// The narrow classes, two workgroups per CU: at most 128 VGPRs (4 waves per SIMD).
+__global__ __launch_bounds__(kPersistWG, 4) void fft_persistent_narrow_kernel(const BatchItem* __restrict__ items,
+                                                                              int n_items,
+                                                                              const BatchTransform* __restrict__ trs,
+                                                                              int* __restrict__ ctl, DeviceTables tb)
+{
+    __shared__ float lds[kNarrowLdsFloats];
+    __shared__ int s_item;
+    persistent_body<true>(items, n_items, trs, ctl, tb, lds, s_item);
+}
+
+// Same as the wide kernel, with the queue in the kernel arguments: a launch that needs no
+// upload.
 __global__ __launch_bounds__(kPersistWG) void fft_persistent_inline_kernel(InlineQueue q, int* __restrict__ ctl,
                                                                             DeviceTables tb)
 {
     __shared__ float lds[kPersistLdsFloats];
     __shared__ int s_item;
-    persistent_body(q.items, q.n_items, q.trs, ctl, tb, lds, s_item);
+    persistent_body<false>(q.items, q.n_items, q.trs, ctl, tb, lds, s_item);
 }
 
 }  // namespace ckfft_gpu
@@ -217,7 +257,7 @@ bool batch_check_desc(const CkFftContext* ctx, const CkFftBatchDesc& d, int inde
 }
 
 bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count, int var_override,
-                 bool check_pointers, BatchBuild& out, TwoPass two_pass)
+                 bool check_pointers, BatchBuild& out, TwoPass two_pass, bool narrow)
 {
     out = BatchBuild{};
     struct Entry {
@@ -251,7 +291,8 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         return a.kind != b.kind ? a.kind < b.kind : a.logc < b.logc;
     });
     out.trs.resize(pers.size());
-    std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item)
+    std::vector<std::pair<long long, BatchItem>> work;   // (bytes, item): the wide kernel's
+    std::vector<std::pair<long long, BatchItem>> work_n; // and the narrow kernel's
     for (size_t i = 0; i < pers.size(); ++i) {
         const CkFftBatchDesc& d = descs[pers[i].idx];
         out.trs[i] = BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr};
@@ -264,16 +305,20 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         size_t j = i;
         while (j < pers.size() && pers[j].kind == e.kind && pers[j].logc == e.logc && (int)(j - i) < tpw) ++j;
         const int cnt = (int)(j - i);
-        work.push_back({cnt * transform_bytes(e.kind, e.log2n), BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
+        (narrow && narrow_class(e.logc, var) && e.logc >= 11 ? work_n : work)
+            .push_back({cnt * transform_bytes(e.kind, e.log2n), BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
         i = j;
     }
-    // largest first, so the tail of the queue is short work
-    std::stable_sort(work.begin(), work.end(),
-                     [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
-                         return a.first > b.first;
-                     });
-    out.items.reserve(work.size());
+    // each queue largest first, so its tail is short work; the wide queue, then the narrow
+    const auto larger = [](const std::pair<long long, BatchItem>& a, const std::pair<long long, BatchItem>& b) {
+        return a.first > b.first;
+    };
+    std::stable_sort(work.begin(), work.end(), larger);
+    std::stable_sort(work_n.begin(), work_n.end(), larger);
+    out.items.reserve(work.size() + work_n.size());
     for (auto& w : work) out.items.push_back(w.second);
+    for (auto& w : work_n) out.items.push_back(w.second);
+    out.n_wide = (int)work.size();
     out.persistent = (int)pers.size();
     return true;
 }
@@ -332,10 +377,11 @@ hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, 
     return hipGetLastError();
 }
 
-int persistent_grid(const CkFftContext* ctx)
+int persistent_grid(const CkFftContext* ctx, bool narrow)
 {
     int per_cu = 0, cus = 0;
-    if (!batch_error(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_persistent_kernel, kPersistWG, 0),
+    if (!batch_error(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu, narrow ? fft_persistent_narrow_kernel : fft_persistent_kernel, kPersistWG, 0),
                      "occupancy") ||
         !batch_error(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count"))
         return 0;
@@ -354,9 +400,10 @@ hipError_t launch_persistent_inline(const BatchBuild& b, int* d_ctl, int grid, c
 }
 
 hipError_t launch_persistent_raw(const BatchItem* d_items, int n_items, const BatchTransform* d_trs, int* d_ctl,
-                                 int grid, const DeviceTables& tb, hipStream_t s)
+                                 int grid, const DeviceTables& tb, hipStream_t s, bool narrow)
 {
-    hipLaunchKernelGGL(fft_persistent_kernel, dim3(grid), dim3(kPersistWG), 0, s, d_items, n_items, d_trs, d_ctl, tb);
+    hipLaunchKernelGGL(narrow ? fft_persistent_narrow_kernel : fft_persistent_kernel, dim3(grid), dim3(kPersistWG), 0,
+                       s, d_items, n_items, d_trs, d_ctl, tb);
     return hipGetLastError();
 }
 
@@ -373,10 +420,11 @@ using namespace ckfft_gpu;
 struct _CkFftBatchPlan {
     CkFftContext* ctx;
     int n_items = 0;
-    int grid = 0;
+    int n_wide = 0;          // items [0, n_wide) for the wide kernel, the rest for the narrow one
+    int grid = 0, grid_narrow = 0;
     BatchItem* d_items = nullptr;
     BatchTransform* d_trs = nullptr;
-    int* d_ctl = nullptr;
+    int* d_ctl = nullptr;    // kCtlWords per persistent launch: wide, narrow
     float2* d_scratch = nullptr;
     BatchTransform* d_pass = nullptr;   // table pass transforms: forward ones, then inverse
     int n_pass_fwd = 0, n_pass_inv = 0;
@@ -395,13 +443,14 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
 {
     if (!ctx || ctx->magic != kContextMagic || count < 0 || (count > 0 && !descs)) return nullptr;
     BatchBuild b;
-    if (!batch_build(ctx, descs, count, var_override, true, b, TwoPass::kTable)) return nullptr;
+    if (!batch_build(ctx, descs, count, var_override, true, b, TwoPass::kTable, true)) return nullptr;
     auto* plan = new (std::nothrow) CkFftBatchPlan();
     if (!plan) return nullptr;
     plan->ctx = ctx;
     plan->use_graph = (flags & kCkFftBatch_Graph) != 0;
     plan->extras = b.extras;
     plan->n_items = (int)b.items.size();
+    plan->n_wide = b.n_wide;
     plan->n_pass_fwd = (int)b.pass_fwd.size();
     plan->n_pass_inv = (int)b.pass_inv.size();
     if (batch_scratch_transforms(b) &&
@@ -420,21 +469,24 @@ CkFftBatchPlan* plan_create(CkFftContext* ctx, const CkFftBatchDesc* descs, int 
     if (plan->n_items) {
         if (!batch_error(hipMalloc(&plan->d_items, sizeof(BatchItem) * b.items.size()), "hipMalloc(items)") ||
             !batch_error(hipMalloc(&plan->d_trs, sizeof(BatchTransform) * b.trs.size()), "hipMalloc(transforms)") ||
-            !batch_error(hipMalloc(&plan->d_ctl, sizeof(int) * kCtlWords), "hipMalloc(control)") ||
+            !batch_error(hipMalloc(&plan->d_ctl, sizeof(int) * 2 * kCtlWords), "hipMalloc(control)") ||
             !batch_error(hipMemcpy(plan->d_items, b.items.data(), sizeof(BatchItem) * b.items.size(),
                                    hipMemcpyHostToDevice), "upload items") ||
             !batch_error(hipMemcpy(plan->d_trs, b.trs.data(), sizeof(BatchTransform) * b.trs.size(),
                                    hipMemcpyHostToDevice), "upload transforms") ||
-            !batch_error(hipMemset(plan->d_ctl, 0, sizeof(int) * kCtlWords), "clear control"))
+            !batch_error(hipMemset(plan->d_ctl, 0, sizeof(int) * 2 * kCtlWords), "clear control"))
             goto bad;
-        const int g = persistent_grid(ctx);
-        if (!g) goto bad;
-        plan->grid = std::min(g, plan->n_items);
+        const int n_narrow = plan->n_items - plan->n_wide;
+        const int g = plan->n_wide ? persistent_grid(ctx, false) : 1;
+        const int gn = n_narrow ? persistent_grid(ctx, true) : 1;
+        if (!g || !gn) goto bad;
+        plan->grid = std::min(g, plan->n_wide);
+        plan->grid_narrow = std::min(gn, n_narrow);
     }
     plan->info.transforms = count;
     plan->info.persistent_transforms = b.persistent;
     plan->info.work_items = plan->n_items;
-    plan->info.workgroups = plan->grid;
+    plan->info.workgroups = plan->grid + plan->grid_narrow;
     plan->info.extra_launches = (int)plan->extras.size();
     plan->info.pass_transforms = plan->n_pass_fwd + plan->n_pass_inv;
     plan->info.algorithmic_bytes = b.bytes;
@@ -444,12 +496,16 @@ bad:
     return nullptr;
 }
 
-// The plan's launches that a graph can hold: the persistent kernel and the table passes.
+// The plan's launches that a graph can hold: the persistent kernels (wide, then narrow)
+// and the table passes.
 hipError_t launch_plan_kernels(CkFftBatchPlan* p, hipStream_t s)
 {
     const DeviceTables tb = context_tables(p->ctx);
     hipError_t e = hipSuccess;
-    if (p->n_items) e = launch_persistent_raw(p->d_items, p->n_items, p->d_trs, p->d_ctl, p->grid, tb, s);
+    if (p->n_wide) e = launch_persistent_raw(p->d_items, p->n_wide, p->d_trs, p->d_ctl, p->grid, tb, s, false);
+    if (e == hipSuccess && p->n_items > p->n_wide)
+        e = launch_persistent_raw(p->d_items + p->n_wide, p->n_items - p->n_wide, p->d_trs, p->d_ctl + kCtlWords,
+                                  p->grid_narrow, tb, s, true);
     if (e == hipSuccess) e = launch_pass_table(p->d_pass, p->n_pass_fwd, p->n_pass_inv, tb, s);
     return e;
 }

@@ -412,13 +412,14 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
 
     float2 v[PL::P];
     if constexpr (T == 1) {
-        // whole transform in this thread's registers (one stage for n <= 16 points)
+        // whole transform in this thread's registers, one stage (n <= 16 points): its
+        // outputs sit in natural order, v[k] = Z[k], which the pack below relies on
+        static_assert(PL::NSTAGES == 1, "a one-thread transform is a single stage");
         if constexpr (KIND == kRealInverse)
             load_first_unpack<PL>(v, t, src, tw);
         else
             load_first_complex<PL>(v, t, src);
         stage_compute<PL, 0, INV>(v, t, tw);
-        run_stages<PL, 1, PL::NSTAGES, INV, SPLIT, -1>(v, t, re, im, tw);
         if constexpr (KIND == kRealForward) {
             if (active) {
                 const float2* lvl = tw + 2 * N;

@@ -153,7 +153,7 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
     // descriptors were validated at submission
     // requests arrive a few at a time: a complex 2^16 one is faster as its own multi-pass
     // launch than as one workgroup's item
-    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b, TwoPass::kExtra)) return false;
+    if (!batch_build(S->ctx, descs.data(), (int)descs.size(), -1, false, b, TwoPass::kExtra, false)) return false;
     const DeviceTables tb = context_tables(S->ctx);
     bool ok = true;
     if (!b.items.empty() && b.items.size() <= (size_t)kInlineQueue && b.trs.size() <= (size_t)kInlineQueue) {
@@ -169,7 +169,7 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
         ok = serr(hipMemcpyAsync(sl.d_desc, sl.h_desc, total, hipMemcpyHostToDevice, sl.stream), "upload queue") &&
              serr(launch_persistent_raw((const BatchItem*)sl.d_desc, (int)b.items.size(),
                                         (const BatchTransform*)(sl.d_desc + trs_off), sl.d_ctl,
-                                        std::min(S->grid, (int)b.items.size()), tb, sl.stream),
+                                        std::min(S->grid, (int)b.items.size()), tb, sl.stream, false),
                   "persistent launch");
     }
     for (const auto& e : b.extras)
@@ -302,7 +302,7 @@ CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxI
         S->resident += (long long)(kDescBytes + sizeof(int) * kCtlWords);
     }
     S->budget = (long long)hbmBudget - S->resident;
-    S->grid = persistent_grid(context);
+    S->grid = persistent_grid(context, false);
     if (S->budget <= 0 || S->grid <= 0) {
         std::snprintf(g_serve_error, sizeof(g_serve_error), "budget below the server's resident %lld bytes",
                       S->resident);

@@ -35,15 +35,16 @@ int CkFftRealForwardBatch(CkFftContext* context, int n, int count,
 int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
                           const CkFftComplex* input, float* output, void* stream);
 
-/* ---- uneven batches in one persistent launch ------------------------------------------
+/* ---- uneven batches in persistent launches --------------------------------------------
  *
  * A batch plan takes any mix of transforms -- each with its own kind, size and device
- * buffers -- and runs them as one launch of a persistent kernel (workgroups pull work
- * items from an on-device queue, largest first).  Complex 2^16 transforms run as two
+ * buffers -- and runs them in persistent kernels (workgroups pull work items from an
+ * on-device queue, largest first): one launch for the classes that need a whole CU's
+ * LDS, one with two workgroups per CU for the rest.  Complex 2^16 transforms run as two
  * table-driven pass launches over all of them, both directions (every workgroup of the
  * GPU on one transform's tiles at a time), and transforms the persistent kernel does not take
  * (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches; all on the same
- * stream after the persistent launch.  CkFftBatchInfo reports how many transforms each
+ * stream after the persistent launches.  CkFftBatchInfo reports how many transforms each
  * route takes.
  * Results are bit-identical to the single-signal calls for every transform.
  * Executions of one plan must not overlap (they share its queue).  The reference has
@@ -65,9 +66,9 @@ typedef struct {
 
 typedef struct {
     int transforms;             /* descriptors in the plan */
-    int persistent_transforms;  /* handled by the persistent launch */
+    int persistent_transforms;  /* handled by the persistent launches */
     int work_items;             /* queue entries */
-    int workgroups;             /* persistent grid size */
+    int workgroups;             /* persistent workgroups, both persistent launches */
     int extra_launches;         /* transforms run as ordinary launches */
     long long algorithmic_bytes;/* one read of every input, one write of every output */
     int pass_transforms;        /* complex 2^16 run by the table-driven pass launches */
@@ -75,7 +76,7 @@ typedef struct {
 
 typedef struct _CkFftBatchPlan CkFftBatchPlan;
 
-/* Flag: capture the plan's launches (the persistent kernel and the table-driven passes)
+/* Flag: capture the plan's launches (the persistent kernels and the table-driven passes)
  * in a hipGraph once and replay it. */
 #define kCkFftBatch_Graph 1
 

@@ -152,6 +152,30 @@ def test_autotune_changes_speed_not_results(torch, C, tmp_path):
             assert c2.load_tuning(path) == 1
 
 
+def test_wide_and_narrow_queues_equal_single_calls(torch, C, tmp_path):
+    """Every class through both persistent kernels: with the 32-points-per-thread layout
+    chosen for every tunable class (a tuning file), the classes up to 2^13 complex points
+    run in the wide kernel and 2^14 in the narrow one; with the default layout they run in
+    the narrow one.  Either way each output equals the single call bit for bit."""
+    path = tmp_path / "all_var1.json"
+    keys = [f'"{k}_{(c + 1) if k[0] == "r" else c}": 1' for k in KINDS for c in range(5, 14)]
+    path.write_text('{\n "arch": "gfx950",\n "choice": {' + ", ".join(keys) + "}\n}\n")
+    mix = [(k, 1 << e) for k in KINDS for e in range(1 if k[0] == "c" else 3, 16 if k[0] == "c" else 17)]
+    for tuned in (False, True):
+        with C.Context(1 << 16) as c:
+            if tuned:
+                assert c.load_tuning(str(path)) == 1
+            ins, outs, descs = build(torch, C, c, mix + mix, 5000)
+            plan = C.BatchPlan(c, descs)
+            assert plan.info()["persistent_transforms"] == len(descs)
+            assert plan.execute() == 1
+            torch.cuda.synchronize()
+            for i, (kind, n) in enumerate(mix + mix):
+                want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
+                assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (tuned, i, kind, n)
+            plan.close()
+
+
 def test_batch_plan_validation(torch, C, ctx):
     x = torch.zeros(64, device="cuda")
     y = torch.zeros(64, device="cuda")
