@@ -238,7 +238,7 @@ class Context:
 
 
 class BatchPlan:
-    """An uneven batch run as one persistent launch (CkFftBatchPlan*).
+    """An uneven batch run in persistent launches (CkFftBatchPlan*).
 
     ``transforms`` is a sequence of (kind, n, input, output) with device buffers.
     """

@@ -42,7 +42,7 @@ int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
  * on-device queue, largest first): one launch for the classes that need a whole CU's
  * LDS, one with two workgroups per CU for the rest.  Complex 2^16 transforms run as two
  * table-driven pass launches over all of them, both directions (every workgroup of the
- * GPU on one transform's tiles at a time), and transforms the persistent kernel does not take
+ * GPU on one transform's tiles at a time), and transforms the persistent kernels do not take
  * (complex n <= 1, real n <= 4, any n above 2^16) run as ordinary launches; all on the same
  * stream after the persistent launches.  CkFftBatchInfo reports how many transforms each
  * route takes.

@@ -1,4 +1,4 @@
-"""GPU: uneven batches in one persistent launch (CkFftBatchPlan*) and the autotuner.
+"""GPU: uneven batches in persistent launches (CkFftBatchPlan*) and the autotuner.
 
 Every transform of a plan must equal, bit for bit, the same transform run alone through
 the single-signal drop-in call (same arithmetic by construction), and match the CPU
