@@ -335,8 +335,10 @@ void batch_assign_scratch(BatchBuild& b, float2* scratch)
 // Complex 2^16 transforms listed in a table (forward ones first, then inverse), each with
 // its own input, output and intermediate: one pass over all of them, both directions in
 // one launch, tiles dealt over the whole GPU.  The same tiles (and so the same results)
-// as the stand-alone multi-pass launches.
-template <bool FIRST>
+// as the stand-alone multi-pass launches.  XTW: where the second pass reads its twiddles,
+// as in pass_twiddles (0: pyramid; 2: the transposed copy, which the host has checked
+// covers level 2^16).
+template <bool FIRST, int XTW>
 __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kernel(
     const BatchTransform* __restrict__ trs, int n_fwd, const float2* __restrict__ pyr,
     const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, long long tiles_per_signal,
@@ -351,12 +353,13 @@ __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kerne
     const float2* src = FIRST ? d.in : d.scratch;
     float2* dst = FIRST ? d.scratch : d.out;
     constexpr int logns = FIRST ? 0 : kPassLogN - kPassLogR;
+    static_assert(XTW == 0 || XTW == 2, "one twiddle source per kernel");
     if (g < n_fwd)   // uniform over the workgroup
-        pass_tile<kPassLogR, SH::LOGCB, false, FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw, lds,
-                                                      threadIdx.x);
+        pass_tile<kPassLogR, SH::LOGCB, false, FIRST, XTW>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
+                                                           lds, threadIdx.x);
     else
-        pass_tile<kPassLogR, SH::LOGCB, true, FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw, lds,
-                                                     threadIdx.x);
+        pass_tile<kPassLogR, SH::LOGCB, true, FIRST, XTW>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
+                                                          lds, threadIdx.x);
 }
 
 hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, const DeviceTables& tb, hipStream_t s)
@@ -368,12 +371,17 @@ hipError_t launch_pass_table(const BatchTransform* d_trs, int n_fwd, int n_inv, 
     const long long tiles = (1ll << (kPassLogN - kPassLogR)) / SH::CB;
     const long long total = tiles * count;
     if (total > 0x7fffffffll) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fft_pass_table_kernel<true>, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd,
-                       tb.pyramid, tb.fine, tb.xtw, tb.log_xtw, tiles, total);
+    const auto first = fft_pass_table_kernel<true, 0>;
+    hipLaunchKernelGGL(first, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd, tb.pyramid, tb.fine, tb.xtw,
+                       tb.log_xtw, tiles, total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fft_pass_table_kernel<false>, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd,
-                       tb.pyramid, tb.fine, tb.xtw, tb.log_xtw, tiles, total);
+    // the second pass has span 2^kLogXtwCols at level 2^16
+    static_assert(kPassLogN - kPassLogR == kLogXtwCols, "second pass span");
+    const auto second = tb.xtw && kPassLogN <= tb.log_xtw ? fft_pass_table_kernel<false, 2>
+                                                          : fft_pass_table_kernel<false, 0>;
+    hipLaunchKernelGGL(second, dim3((unsigned)total), dim3(SH::WG), 0, s, d_trs, n_fwd, tb.pyramid, tb.fine, tb.xtw,
+                       tb.log_xtw, tiles, total);
     return hipGetLastError();
 }
 

@@ -64,9 +64,9 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
 
 // Inter-pass twiddles of butterfly column jm in a pass of span 2^logns: the thread's point r
 // (r = in_index of its first-stage element) is multiplied by W_L^(r jm), L = 2^logns * R.
-// XTW: 0 = twiddles from the pyramid / fine tables only; 1 = from the transposed copy
-// when it covers this pass (span 2^kLogXtwCols, level <= 2^log_xtw), checked at run time;
-// 2 = from the transposed copy (the caller has checked that it covers the pass).
+// XTW: 0 = twiddles from the pyramid / fine tables; 2 = from the transposed copy, which
+// the caller has checked covers the pass (span 2^kLogXtwCols, level <= 2^log_xtw).  The
+// host picks one per kernel, so a kernel has a single code path.
 template <class PL, bool INV, int XTW>
 __device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, int logns,
                                               const float2* __restrict__ pyr, const float2* __restrict__ fine,
@@ -77,7 +77,8 @@ __device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, i
     constexpr int R0 = 1 << PL::rlog(0);
     constexpr int B0 = PL::P / R0;
     const int loglev = logns + LOGR;
-    if (XTW == 2 || (XTW == 1 && xtw && logns == kLogXtwCols && loglev <= log_xtw)) {
+    static_assert(XTW == 0 || XTW == 2, "twiddle source");
+    if constexpr (XTW == 2) {
         // the same values from the transposed copy: neighbouring columns, neighbouring words
         const float2* tcol = xtw + jm;
         const int sh = log_xtw - loglev;
@@ -88,7 +89,7 @@ __device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, i
                 const int r = in_index<PL, 0>(tA + b * T, rr);
                 v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
             }
-    } else if constexpr (XTW != 2) {
+    } else {
         // The thread's points are r = tA + b*T + rr*M0 (M0 = R/R0), so
         // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: B0 + log2(R0) table reads per
         // thread instead of one per point.  Above 2^16 every read gathers from a fine
@@ -116,7 +117,7 @@ __device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, i
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
 // `lds` holds CB * RSTRIDE floats.  Used by the pass kernels here and by the batch
 // plans' table-driven passes.  XTW as in pass_twiddles.
-template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW = 1>
+template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW>
 __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
                                           int logn, int logns, const float2* __restrict__ pyr,
                                           const float2* __restrict__ fine, const float2* __restrict__ xtw,
