@@ -77,6 +77,23 @@ def test_serving_alone_equals_mix(torch, C, ctx):
     assert_alone_equals_mix(torch, ctx, mix, ins, outs)
 
 
+def test_serving_tuned_layouts_alone_equals_mix(torch, C, tmp_path):
+    """With the 32-points-per-thread layout chosen for every tunable class (a tuning file),
+    every class from 2 to 2^16 points served in a Poisson stream equals the single call."""
+    path = tmp_path / "all_var1.json"
+    keys = [f'"{k}_{(c + 1) if k[0] == "r" else c}": 1' for k in KINDS for c in range(5, 14)]
+    path.write_text('{\n "arch": "gfx950",\n "choice": {' + ", ".join(keys) + "}\n}\n")
+    mix = [(k, 1 << e) for k in KINDS for e in range(1 if k[0] == "c" else 3, 17)] * 3
+    with C.Context(1 << 16) as c:
+        assert c.load_tuning(str(path)) == 1
+        ins, outs, descs = build(torch, C, c, mix, 9000)
+        torch.cuda.synchronize()
+        with C.Server(c, 1 << 30, max_in_flight=4) as srv:
+            srv.replay(descs, poisson_arrivals(len(descs), 200e3, 54))
+            assert srv.drain() == 1
+        assert_alone_equals_mix(torch, c, mix, ins, outs)
+
+
 def test_serving_against_oracle(torch, C, ctx):
     mix = workload_mix(40, 53, kmin=3, kmax=16)
     ins, outs, descs = build(torch, C, ctx, mix, 8000)
