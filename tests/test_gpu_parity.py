@@ -291,6 +291,38 @@ def test_device_arena_context(torch, C):
     assert size.value > 0                               # query mode reports the host part
 
 
+def test_device_arena_multipass_plan_and_server(torch, C):
+    """An arena context at nMax = 2^20 (pyramid, fine tables and the transposed pass table
+    all in the arena): multi-pass transforms of every kind, a batch plan and a server give
+    the bits of a library-allocated context."""
+    n = 1 << 20
+    need = C.CkFftDeviceTableBytes(n, C.kCkFftDirection_Both)
+    arena = torch.zeros(need, dtype=torch.uint8, device="cuda")
+    cases = [(k, n >> s) for k in ("cfwd", "cinv", "rfwd", "rinv") for s in (0, 3)]
+    ins = {c: make_input(c[0], c[1], 1, 6500 + i) for i, c in enumerate(cases)}
+    with C.Context(n) as lib_ctx:
+        want = {c: run_batch(torch, lib_ctx, c[0], c[1], ins[c]) for c in cases}
+    with C.Context(n, C.kCkFftDirection_Both, arena=arena) as ac:
+        for c in cases:
+            assert np.array_equal(run_batch(torch, ac, c[0], c[1], ins[c]).view(np.uint32),
+                                  want[c].view(np.uint32)), c
+        kinds = ("cfwd", "cinv", "rfwd", "rinv")
+        xs = [dev(torch, ins[c][0]) for c in cases]
+        for route in ("plan", "server"):
+            ys = [torch.full((out_len(k, m),), float("nan"), dtype=torch.float32, device="cuda") for k, m in cases]
+            descs = [(kinds.index(k), m, xs[i], ys[i]) for i, (k, m) in enumerate(cases)]
+            if route == "plan":
+                plan = C.BatchPlan(ac, descs)
+                assert plan.execute() == 1
+                plan.close()
+            else:
+                with C.Server(ac, 1 << 30) as srv:
+                    assert all(srv.wait(srv.submit(*d)) == 1 for d in descs)
+            torch.cuda.synchronize()
+            for i, c in enumerate(cases):
+                assert np.array_equal(ys[i].cpu().numpy().view(np.uint32), want[c][0].view(np.uint32)), (route, c)
+
+
 def test_max_count_does_not_change_results(torch, C):
     """Like the reference (float twiddle angle scaling by 2^j is exact), results are
     identical for any nMax >= n."""
