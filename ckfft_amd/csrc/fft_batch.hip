@@ -1,26 +1,30 @@
-// fft_batch.hip -- uneven batches in one persistent launch, and the variant autotuner.
+// fft_batch.hip -- uneven batches in persistent launches, and the variant autotuner.
 //
 // The reference runs one transform per call on the calling thread
 // (src/ckfft/ckfft.cpp:36-114); it has no batching.  Here a batch of transforms of any
-// mix of sizes, kinds and directions (each with its own device buffers) runs as ONE
-// launch of a persistent kernel:
+// mix of sizes, kinds and directions (each with its own device buffers) runs as two
+// launches of persistent kernels:
 //  * the host groups the transforms into work items -- a group of same-class transforms
-//    that fills one 512-thread workgroup -- and orders them largest-first;
-//  * one workgroup per CU loops: take the next item from an atomic queue, run it with
+//    that fills one 512-thread workgroup -- and puts each item in one of two queues,
+//    each ordered largest-first: the wide kernel's (LDS for any class, one workgroup per
+//    CU) and the narrow kernel's (the classes that fit half of that, two workgroups per
+//    CU, so that one item's loads overlap another's arithmetic);
+//  * each workgroup loops: take the next item from its kernel's atomic queue, run it with
 //    the same transform body the batch kernels use (fft_lds.h).  Items never wait on
-//    each other, so the kernel needs no cross-workgroup synchronisation;
-//  * the last workgroup to leave resets the queue, so a plan re-executes (eagerly or as
+//    each other, so the kernels need no cross-workgroup synchronisation;
+//  * the last workgroup to leave resets its queue, so a plan re-executes (eagerly or as
 //    a replayed hipGraph) with no other launch.
-// Complex 2^16 transforms (above one workgroup's LDS) run after it as two table-driven
+// Complex 2^16 transforms (above one workgroup's LDS) run after them as two table-driven
 // pass launches over all of them (both directions), with the tiles of the stand-alone
-// multi-pass kernels; transforms neither takes (closed-form sizes, sizes above 2^16) run
+// multi-pass kernels; transforms none takes (closed-form sizes, sizes above 2^16) run
 // through the ordinary launches, on the same stream.
 //
-// Autotuning: every class (kind, size) may run with 16 or 32 points per thread, which
-// changes waves per transform, transforms per workgroup and LDS tile, but never the
-// arithmetic (the radix plan and every twiddle are the same), so a tuned plan gives
-// bit-identical results to the single-signal calls.  CkFftAutotune times both variants
-// per class and keeps the faster in the context and, optionally, in a JSON cache.
+// Autotuning: the classes of 2^6 to 2^13 complex points may run with 16 or 32 points per
+// thread, which changes waves per transform, transforms per workgroup, LDS tile and
+// queue (32 points per thread runs in the wide kernel), but never the arithmetic (the
+// radix plan and every twiddle are the same), so a tuned plan gives bit-identical
+// results to the single-signal calls.  CkFftAutotune times both variants per class and
+// keeps the faster in the context and, optionally, in a JSON cache.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
