@@ -340,7 +340,7 @@ void batch_assign_scratch(BatchBuild& b, float2* scratch)
 // its own input, output and intermediate: one pass over all of them, both directions in
 // one launch, tiles dealt over the whole GPU.  The same tiles (and so the same results)
 // as the stand-alone multi-pass launches.  XTW: where the second pass reads its twiddles,
-// as in pass_twiddles (0: pyramid; 2: the transposed copy, which the host has checked
+// as in PassTwiddles (0: pyramid; 2: the transposed copy, which the host has checked
 // covers level 2^16).
 template <bool FIRST, int XTW>
 __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kernel(

@@ -67,56 +67,97 @@ __device__ __forceinline__ void pass_stages(float2* v, int tA, float* reA, float
 // XTW: 0 = twiddles from the pyramid / fine tables; 2 = from the transposed copy, which
 // the caller has checked covers the pass (span 2^kLogXtwCols, level <= 2^log_xtw).  The
 // host picks one per kernel, so a kernel has a single code path.
+// fetch() issues the table reads and apply() forms the twiddles and multiplies the points,
+// so that a tile can issue the reads before its data loads and their latency overlaps
+// (at level 2^24 the reads cost as much as 40% of the pass when issued after the data).
+// The values and products are those of reading each twiddle where it is used.
 template <class PL, bool INV, int XTW>
-__device__ __forceinline__ void pass_twiddles(float2* v, int tA, long long jm, int logns,
-                                              const float2* __restrict__ pyr, const float2* __restrict__ fine,
-                                              const float2* __restrict__ xtw, int log_xtw)
-{
-    constexpr int LOGR = PL::nslog(PL::NSTAGES);              // log2 of the pass radix
-    constexpr int R = PL::N, T = PL::T;
-    constexpr int R0 = 1 << PL::rlog(0);
-    constexpr int B0 = PL::P / R0;
-    const int loglev = logns + LOGR;
+struct PassTwiddles {
     static_assert(XTW == 0 || XTW == 2, "twiddle source");
-    if constexpr (XTW == 2) {
-        // the same values from the transposed copy: neighbouring columns, neighbouring words
-        const float2* tcol = xtw + jm;
-        const int sh = log_xtw - loglev;
+    static constexpr int LOGR = PL::nslog(PL::NSTAGES);       // log2 of the pass radix
+    static constexpr int R = PL::N, T = PL::T;
+    static constexpr int LR0 = PL::rlog(0);
+    static constexpr int R0 = 1 << LR0;
+    static constexpr int B0 = PL::P / R0;
+    static constexpr int M0 = R / R0;
+    // XTW 2: one value per point.  XTW 0: the thread's points are r = tA + b*T + rr*M0, so
+    // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: log2(R0) base powers, then B0 row
+    // values, read once per thread instead of once per point.  Above 2^16 each value is a
+    // coarse pyramid entry times a fine-table entry; gathering one per point made the pass
+    // bound by the texture-address unit.
+    static constexpr int NV = XTW == 2 ? PL::P : LR0 + B0;
+    float2 f[NV];                       // pyramid, fine-table or transposed entries
+    float2 c[XTW == 2 ? 1 : NV];        // coarse pyramid entries (levels above 2^16)
+    bool coarse;
+
+    __device__ __forceinline__ void fetch(int tA, long long jm, int logns, const float2* __restrict__ pyr,
+                                          const float2* __restrict__ fine, const float2* __restrict__ xtw,
+                                          int log_xtw)
+    {
+        const int loglev = logns + LOGR;
+        if constexpr (XTW == 2) {
+            // neighbouring columns, neighbouring words
+            const float2* tcol = xtw + jm;
+            const int sh = log_xtw - loglev;
 #pragma unroll
-        for (int b = 0; b < B0; ++b)
+            for (int b = 0; b < B0; ++b)
 #pragma unroll
-            for (int rr = 0; rr < R0; ++rr) {
-                const int r = in_index<PL, 0>(tA + b * T, rr);
-                v[b * R0 + rr] = cmul(v[b * R0 + rr], dirw<INV>(tcol[(long long)(r << sh) << kLogXtwCols]));
+                for (int rr = 0; rr < R0; ++rr) {
+                    const int r = in_index<PL, 0>(tA + b * T, rr);
+                    f[b * R0 + rr] = tcol[(long long)(r << sh) << kLogXtwCols];
+                }
+        } else {
+            long long m[NV];
+#pragma unroll
+            for (int k = 0; k < LR0; ++k) m[k] = (long long)(M0 << k) * jm;
+#pragma unroll
+            for (int b = 0; b < B0; ++b) m[LR0 + b] = (long long)(tA + b * T) * jm;
+            coarse = loglev > kLogMaxPyramid;
+            if (coarse) {
+                const float2* cl = pyr + (1ll << (loglev - kLogFine));
+                const float2* fl = fine + ((long long)(loglev - kLogMaxPyramid - 1) << kLogFine);
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    c[i] = cl[m[i] >> kLogFine];
+                    f[i] = fl[m[i] & ((1 << kLogFine) - 1)];
+                }
+            } else {
+                const float2* lvl = pyr + (1ll << loglev);
+#pragma unroll
+                for (int i = 0; i < NV; ++i) f[i] = lvl[m[i]];
             }
-    } else {
-        // The thread's points are r = tA + b*T + rr*M0 (M0 = R/R0), so
-        // W^(r jm) = W^((tA + b T) jm) * (W^(M0 jm))^rr: B0 + log2(R0) table reads per
-        // thread instead of one per point.  Above 2^16 every read gathers from a fine
-        // table (and multiplies by a coarse entry); gathering one per point made the
-        // pass bound by the texture-address unit.
-        constexpr int M0 = R / R0;
-        TwBase<R0> sb;
-        sb.w[0] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)M0 * jm));
-        if constexpr (R0 >= 4) sb.w[1] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(2 * M0) * jm));
-        if constexpr (R0 >= 8) sb.w[2] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(4 * M0) * jm));
-        if constexpr (R0 >= 16) sb.w[3] = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(8 * M0) * jm));
-        float2 step[R0];
-        twiddle_powers<R0>(step, sb);
-#pragma unroll
-        for (int b = 0; b < B0; ++b) {
-            const float2 wb = dirw<INV>(twiddle_at(pyr, fine, loglev, (long long)(tA + b * T) * jm));
-            v[b * R0] = cmul(v[b * R0], wb);
-#pragma unroll
-            for (int rr = 1; rr < R0; ++rr) v[b * R0 + rr] = cmul(v[b * R0 + rr], cmul(wb, step[rr]));
         }
     }
-}
+
+    // entry i as twiddle_at forms it
+    __device__ __forceinline__ float2 value(int i) const { return coarse ? cmul(c[i], f[i]) : f[i]; }
+
+    __device__ __forceinline__ void apply(float2* v) const
+    {
+        if constexpr (XTW == 2) {
+#pragma unroll
+            for (int i = 0; i < PL::P; ++i) v[i] = cmul(v[i], dirw<INV>(f[i]));
+        } else {
+            TwBase<R0> sb;
+#pragma unroll
+            for (int k = 0; k < LR0; ++k) sb.w[k] = dirw<INV>(value(k));
+            float2 step[R0];
+            twiddle_powers<R0>(step, sb);
+#pragma unroll
+            for (int b = 0; b < B0; ++b) {
+                const float2 wb = dirw<INV>(value(LR0 + b));
+                v[b * R0] = cmul(v[b * R0], wb);
+#pragma unroll
+                for (int rr = 1; rr < R0; ++rr) v[b * R0 + rr] = cmul(v[b * R0 + rr], cmul(wb, step[rr]));
+            }
+        }
+    }
+};
 
 // One tile of one pass: butterflies q*CB .. q*CB + CB-1 of the signal at `src` (2^logn
 // points), written to `dst`; logns = log2 of the pass span (0 for the first pass).
 // `lds` holds CB * RSTRIDE floats.  Used by the pass kernels here and by the batch
-// plans' table-driven passes.  XTW as in pass_twiddles.
+// plans' table-driven passes.  XTW as in PassTwiddles.
 template <int LOGR, int LOGCB, bool INV, bool FIRST, int XTW>
 __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2* __restrict__ dst, long long q,
                                           int logn, int logns, const float2* __restrict__ pyr,
@@ -140,6 +181,8 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
     float2 v[PL::P];
     constexpr int R0 = 1 << PL::rlog(0);
     constexpr int B0 = PL::P / R0;
+    PassTwiddles<PL, INV, XTW> ptw;
+    if constexpr (!FIRST) ptw.fetch(tA, (j0 + cA) & ((1ll << logns) - 1), logns, pyr, fine, xtw, log_xtw);
 #pragma unroll
     for (int b = 0; b < B0; ++b)
 #pragma unroll
@@ -147,8 +190,7 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
             const int r = in_index<PL, 0>(tA + b * T, rr);
             v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
         }
-    if constexpr (!FIRST)
-        pass_twiddles<PL, INV, XTW>(v, tA, (j0 + cA) & ((1ll << logns) - 1), logns, pyr, fine, xtw, log_xtw);
+    if constexpr (!FIRST) ptw.apply(v);
     stage_compute<PL, 0, INV>(v, tA, pyr);
     pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
 

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_unpack_kernel(
     const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;
     const int cB = tid / T, tB = tid - cB * T;
     const long long col = mirror_col(q, cA, CB, ncols);
+    const float2 wcol = twiddle_at(pyr, fine, logn, col);   // W_2h^col, read ahead of the data
 
     float2 v[PL::P];
 #pragma unroll
@@ -115,7 +116,6 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_unpack_kernel(
         }
     __syncthreads();
     const bool self = col == 0 || col == ncols / 2;
-    const float2 wcol = twiddle_at(pyr, fine, logn, col);   // W_2h^col
     const float* mrow = lds + (self ? cA : (cA < CB / 2 ? cA + CB / 2 : cA - CB / 2)) * SH::RSTRIDE;
 #pragma unroll
     for (int b = 0; b < B0; ++b)
@@ -184,11 +184,13 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_pack_kernel(
     const long long col = mirror_col(q, cA, CB, ncols);
 
     float2 v[PL::P];
+    PassTwiddles<PL, false, XTW_PASS ? 2 : 0> ptw;
+    ptw.fetch(tA, col, logns, pyr, fine, xtw, log_xtw);
 #pragma unroll
     for (int b = 0; b < B0; ++b)
 #pragma unroll
         for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = zs[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols];
-    pass_twiddles<PL, false, XTW_PASS ? 2 : 0>(v, tA, col, logns, pyr, fine, xtw, log_xtw);
+    ptw.apply(v);
     float* reA = lds + cA * SH::RSTRIDE;
     float* imA = reA + SH::SPAN;
     float* reB = lds + cB * SH::RSTRIDE;
