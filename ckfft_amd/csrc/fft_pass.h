@@ -96,22 +96,22 @@ struct PassTwiddles {
     {
         const int loglev = logns + LOGR;
         if constexpr (XTW == 2) {
-            // neighbouring columns, neighbouring words
-            const float2* tcol = xtw + jm;
-            const int sh = log_xtw - loglev;
+            // neighbouring columns, neighbouring words; point r's row starts at word
+            // r << sh, and r = tA + in_index(b*T, rr) (in_index is linear in the butterfly)
+            const int sh = log_xtw - loglev + kLogXtwCols;
+            const float2* tcol = xtw + jm + ((long long)tA << sh);
 #pragma unroll
             for (int b = 0; b < B0; ++b)
 #pragma unroll
-                for (int rr = 0; rr < R0; ++rr) {
-                    const int r = in_index<PL, 0>(tA + b * T, rr);
-                    f[b * R0 + rr] = tcol[(long long)(r << sh) << kLogXtwCols];
-                }
+                for (int rr = 0; rr < R0; ++rr) f[b * R0 + rr] = tcol[in_index<PL, 0>(b * T, rr) << sh];
         } else {
-            long long m[NV];
+            // every exponent is below the level L <= 2^28: 32-bit arithmetic
+            const unsigned jmu = (unsigned)jm;
+            unsigned m[NV];
 #pragma unroll
-            for (int k = 0; k < LR0; ++k) m[k] = (long long)(M0 << k) * jm;
+            for (int k = 0; k < LR0; ++k) m[k] = (unsigned)(M0 << k) * jmu;
 #pragma unroll
-            for (int b = 0; b < B0; ++b) m[LR0 + b] = (long long)(tA + b * T) * jm;
+            for (int b = 0; b < B0; ++b) m[LR0 + b] = (unsigned)(tA + b * T) * jmu;
             coarse = loglev > kLogMaxPyramid;
             if (coarse) {
                 const float2* cl = pyr + (1ll << (loglev - kLogFine));
@@ -168,7 +168,6 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
     using PL = Plan<LOGR, SH::LOGP>;
     static_assert(PL::NSTAGES >= 2, "a pass needs at least two sub-stages");
     constexpr int R = SH::R, T = SH::T, CB = SH::CB;
-    const long long stride = (1ll << logn) >> LOGR;         // N/R
     const long long j0 = q * CB;
 
     const int cA = tid & (CB - 1), tA = tid >> SH::LOGCB;  // DFT-fastest lanes
@@ -183,13 +182,14 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
     constexpr int B0 = PL::P / R0;
     PassTwiddles<PL, INV, XTW> ptw;
     if constexpr (!FIRST) ptw.fetch(tA, (j0 + cA) & ((1ll << logns) - 1), logns, pyr, fine, xtw, log_xtw);
+    // point r = tA + in_index(b*T, rr) of column j0 + cA: a per-thread base plus offsets
+    // that are the same for every thread
+    const int logstride = logn - LOGR;
+    const float2* srct = src + j0 + cA + ((long long)tA << logstride);
 #pragma unroll
     for (int b = 0; b < B0; ++b)
 #pragma unroll
-        for (int rr = 0; rr < R0; ++rr) {
-            const int r = in_index<PL, 0>(tA + b * T, rr);
-            v[b * R0 + rr] = src[j0 + cA + (long long)r * stride];
-        }
+        for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = srct[(long long)in_index<PL, 0>(b * T, rr) << logstride];
     if constexpr (!FIRST) ptw.apply(v);
     stage_compute<PL, 0, INV>(v, tA, pyr);
     pass_stages<PL, 1, INV, FIRST, SH>(v, tA, reA, imA, tB, reB, imB, pyr);
@@ -212,7 +212,7 @@ __device__ __forceinline__ void pass_tile(const float2* __restrict__ src, float2
         for (int b = 0; b < BL; ++b)
 #pragma unroll
             for (int rr = 0; rr < RL; ++rr)
-                base[(long long)out_index<PL, SL>(tA + b * T, rr) * ns] = v[b * RL + rr];
+                base[(unsigned)out_index<PL, SL>(tA + b * T, rr) << logns] = v[b * RL + rr];   // < N <= 2^28
     }
 }
 
