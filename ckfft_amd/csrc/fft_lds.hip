@@ -15,7 +15,9 @@ __global__ __launch_bounds__(WG) void fft_lds_kernel(const float2* __restrict__ 
 {
     using SH = LdsShape<LOGN, LOGP, WG, KIND, SPLIT>;
     __shared__ float lds[SH::LDS_WORDS];
-    const int tr = threadIdx.x / SH::T;
+    // one transform per workgroup: its index is uniform, so the signal's base address
+    // stays in scalar registers
+    const int tr = SH::TPW == 1 ? 0 : (int)threadIdx.x / SH::T;
     const long long g = (long long)blockIdx.x * SH::TPW + tr;
     const bool active = g < count;
     // slots past the last signal (only in the last workgroup) compute on the last
