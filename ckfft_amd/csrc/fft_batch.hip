@@ -80,7 +80,8 @@ __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTran
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     __builtin_assume(tid >= 0 && tid < kPersistWG);
-    const int slot = tid / SH::T;
+    // a class of one transform per workgroup has one slot: its buffers are uniform
+    const int slot = SH::TPW == 1 ? 0 : tid / SH::T;
     const bool active = slot < w.count;
     const BatchTransform d = trs[w.first + (active ? slot : 0)];
     lds_transform<LOGC, LP, kPersistWG, KIND, SPLIT>(d.in, d.out, active, tw, lds, tid);
