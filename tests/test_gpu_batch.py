@@ -176,6 +176,35 @@ def test_wide_and_narrow_queues_equal_single_calls(torch, C, tmp_path):
             plan.close()
 
 
+def test_queue_routing_by_class(torch, C):
+    """Classes up to 2^13 complex points at 16 points per thread, and 2^14, go to the narrow
+    kernel (two workgroups per CU); 2^15 complex and 2^16 real go to the wide one (one per
+    CU).  The grid sizes CkFftBatchInfo reports follow the items of each queue."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    # (transforms, wide grid, narrow grid); a grid is min(items, CUs x workgroups per CU), an
+    # item holds 512 / (n / points per thread) transforms of one class
+    cases = [
+        ([("cfwd", 1 << 12)] * 3 * cus, 0, min(3 * cus // 2, 2 * cus)),  # 2 per item, narrow
+        ([("rinv", 1 << 15)] * 3 * cus, 0, 2 * cus),                     # real 2^15: 2^14 points, narrow
+        ([("cinv", 1 << 15)] * 3 * cus, cus, 0),                         # wide only
+        ([("rfwd", 1 << 16)] * 5, 5, 0),
+        ([("cfwd", 1 << 15)] * 3 + [("cfwd", 1 << 10)] * 8, 3, 1),       # 8 transforms of 2^10: one item
+    ]
+    with C.Context(1 << 16) as c:
+        for mix, wide, narrow in cases:
+            ins, outs, descs = build(torch, C, c, mix, 9100)
+            plan = C.BatchPlan(c, descs)
+            info = plan.info()
+            assert info["workgroups"] == wide + narrow, (mix[0], info)
+            assert plan.execute() == 1
+            torch.cuda.synchronize()
+            for i in (0, len(mix) - 1):
+                kind, n = mix[i]
+                want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
+                assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
+            plan.close()
+
+
 def test_batch_plan_validation(torch, C, ctx):
     x = torch.zeros(64, device="cuda")
     y = torch.zeros(64, device="cuda")
