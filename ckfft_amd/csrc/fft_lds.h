@@ -284,10 +284,40 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
     if (active && t == 0) dst[N] = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
 }
 
+// The signal base of a workgroup that owns one transform, in scalar registers.
+__device__ __forceinline__ float2* uniform_base(float2* p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return (float2*)(((unsigned long long)hi << 32) | lo);
+}
+
+typedef unsigned StoreBits __attribute__((ext_vector_type(2)));
+
+// Every output index of stage S (no mirror) equals IndexSplit's U(t) + C(b, r) and lies in
+// [0, N): checked for every thread, butterfly and element at compile time.
+template <class PL, int S>
+constexpr bool store_split_exact()
+{
+    using IS = IndexSplit<PL, S, false, true>;
+    for (int t = 0; t < PL::T; ++t)
+        for (int b = 0; b < IS::B; ++b)
+            for (int r = 0; r < IS::R; ++r) {
+                const int i = IS::U(t, 0) + IS::C(b, r);
+                if (i != IS::reference(t, b, r) || i < 0 || i >= PL::N) return false;
+            }
+    return true;
+}
+
 // The last stage of a complex or real-inverse transform fused with its stores, butterfly
 // by butterfly.  The stage's twiddles (tws) are read beforehand: a wait for a read issued
 // after the first stores would also wait for those stores.
-template <class PL, bool INV>
+// UNIFORM (one transform per workgroup): the stores go through a buffer resource over the
+// signal, each index split as IndexSplit's per-thread part (a VGPR byte offset, computed
+// once) plus its compile-time part (a scalar offset), so no store needs 64-bit address
+// arithmetic.  The resource spans the signal's N points; every index is below N.
+template <class PL, bool INV, bool UNIFORM>
 __device__ __forceinline__ void last_stage_store(float2* v, int t,
                                                  StageTwiddles<PL, PL::NSTAGES - 1, INV, false>& tws,
                                                  float2* __restrict__ dst, bool active)
@@ -296,12 +326,31 @@ __device__ __forceinline__ void last_stage_store(float2* v, int t,
     constexpr int R = 1 << PL::rlog(S);
     constexpr int B = PL::P / R;
     tws.settle();
+    if constexpr (UNIFORM) {
+        using IS = IndexSplit<PL, S, false, true>;
+        static_assert(store_split_exact<PL, S>(), "store index split");
+        constexpr int kRawBufferFormat = 0x00020000;   // 32-bit data format, untyped access
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(uniform_base(dst), (short)0, PL::N * (int)sizeof(float2), kRawBufferFormat);
+        const unsigned vo = (unsigned)IS::U(t, 0) * (unsigned)sizeof(float2);
 #pragma unroll
-    for (int b = 0; b < B; ++b) {
-        butterfly_compute<PL, S, INV, false>(v, tws, b);
-        if (active) {
+        for (int b = 0; b < B; ++b) {
+            butterfly_compute<PL, S, INV, false>(v, tws, b);
+            if (active) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
+                for (int r = 0; r < R; ++r)
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, v[b * R + r]), rs, vo,
+                                                          IS::C(b, r) * (int)sizeof(float2), 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            butterfly_compute<PL, S, INV, false>(v, tws, b);
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
+            }
         }
     }
 }
@@ -468,7 +517,7 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
         } else {
             StageTwiddles<PL, SL, INV, false> tws(t, tw);
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
-            last_stage_store<PL, INV>(v, t, tws, dst, active);
+            last_stage_store<PL, INV, SH::TPW == 1>(v, t, tws, dst, active);
         }
     }
 }
