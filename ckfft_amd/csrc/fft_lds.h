@@ -231,6 +231,19 @@ __device__ __forceinline__ void pack_pair(int k, float2 zk, float2 zm, const flo
     if (k != N - k) dst[N - k] = pack_one(zm, zk, edge_twiddle<N, R>(lvl, N - k));
 }
 
+// The signal base of a workgroup that owns one transform, in scalar registers.
+__device__ __forceinline__ float2* uniform_base(float2* p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return (float2*)(((unsigned long long)hi << 32) | lo);
+}
+
+typedef unsigned StoreBits __attribute__((ext_vector_type(2)));
+// Buffer resource word 3 for untyped accesses: 32-bit data format.
+constexpr int kRawBufferFormat = 0x00020000;
+
 // Real forward: the mirror-closed last stage fused with the pack.  Every output point k
 // is written once, by the thread that holds Z[k], from Z[k] and its mirror Z[N-k] in the
 // same thread's registers.  Thread 0's first slot pair holds the self-mirrored
@@ -238,7 +251,37 @@ __device__ __forceinline__ void pack_pair(int k, float2 zk, float2 zm, const flo
 // out[N].  Pair by pair (slots b and b+H): the two butterflies compute, then their
 // outputs are packed and stored.  The stage's twiddles (tws) and the pack twiddles W^jl
 // of the lower slots (wl) are read beforehand.
-template <class PL, bool INV>
+// The pack's store indices as per-thread parts plus non-negative compile-time parts, and
+// inside the N+1 outputs: k0 = t + (b T + r M); k1 = (M - t - (H-1) T) + ((H-1-b) T +
+// (R-1-r) M) for b > 0 and jh0 + (R-1-r) M for b == 0.  Checked for every thread.
+template <class PL>
+constexpr bool pack_split_exact()
+{
+    constexpr int S = PL::NSTAGES - 1, N = PL::N, R = 1 << PL::rlog(S), B = PL::P / R, H = B / 2, M = N / R;
+    constexpr int T = PL::T;
+    for (int t = 0; t < T; ++t) {
+        const int jh0 = t == 0 ? M / 2 : M - t;
+        const int um = M - t - (H - 1) * T;
+        if (um < 0) return false;
+        for (int b = 0; b < H; ++b)
+            for (int r = 0; r < R; ++r) {
+                const int jl = t + b * T, jh = b == 0 ? jh0 : M - jl;
+                const int k0 = jl + r * M, k1 = jh + (R - 1 - r) * M;
+                const int c0 = b * T + r * M;
+                const int c1 = (H - 1 - b) * T + (R - 1 - r) * M, c1b0 = (R - 1 - r) * M;
+                if (t + c0 != k0 || k0 < 0 || k0 > N || c0 < 0) return false;
+                if (b == 0 && (jh0 + c1b0 != k1 || c1b0 < 0)) return false;
+                if (b > 0 && (um + c1 != k1 || c1 < 0)) return false;
+                if (k1 < 0 || k1 > N) return false;
+            }
+    }
+    return true;
+}
+
+// UNIFORM (one transform per workgroup): the stores go through a buffer resource over the
+// N+1 outputs, per-thread byte offsets in VGPRs and the compile-time parts as scalar
+// offsets (pack_split_exact), so no store needs 64-bit address arithmetic.
+template <class PL, bool INV, bool UNIFORM>
 __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const float2* __restrict__ tw,
                                                        StageTwiddles<PL, PL::NSTAGES - 1, INV, true>& tws,
                                                        float2 (&wl)[PL::P / (1 << PL::rlog(PL::NSTAGES - 1)) / 2],
@@ -252,6 +295,16 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
     constexpr int M = N / R;                    // the last stage writes point j + r*M
     const float2* lvl = tw + 2 * N;
     const int jh0 = bfly<PL, S, true>(t, H);    // M - t, or M/2 for t == 0
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned vo0 = 0, voh0 = 0, vom = 0;
+    if constexpr (UNIFORM) {
+        static_assert(pack_split_exact<PL>(), "pack store index split");
+        rs = __builtin_amdgcn_make_buffer_rsrc(uniform_base(dst), (short)0, (N + 1) * (int)sizeof(float2),
+                                               kRawBufferFormat);
+        vo0 = (unsigned)t * (unsigned)sizeof(float2);
+        voh0 = (unsigned)jh0 * (unsigned)sizeof(float2);
+        vom = (unsigned)(M - t - (H - 1) * PL::T) * (unsigned)sizeof(float2);
+    }
     tws.settle();
     settle_reads(wl);
 #pragma unroll
@@ -277,23 +330,21 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
                 p1 = z ? v[H * R + r] : p1;
                 w1 = z ? w64((2 * (R - 1 - r) + 1) * (16 / R)) : w1;
             }
-            dst[k0] = pack_one(lo, p0, w0);
-            dst[k1] = pack_one(hi, p1, w1);
+            if constexpr (UNIFORM) {
+                const int c0 = b * PL::T + r * M;
+                const int c1 = b == 0 ? (R - 1 - r) * M : (H - 1 - b) * PL::T + (R - 1 - r) * M;
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, pack_one(lo, p0, w0)), rs, vo0,
+                                                      c0 * (int)sizeof(float2), 0);
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, pack_one(hi, p1, w1)), rs,
+                                                      b == 0 ? voh0 : vom, c1 * (int)sizeof(float2), 0);
+            } else {
+                dst[k0] = pack_one(lo, p0, w0);
+                dst[k1] = pack_one(hi, p1, w1);
+            }
         }
     }
     if (active && t == 0) dst[N] = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
 }
-
-// The signal base of a workgroup that owns one transform, in scalar registers.
-__device__ __forceinline__ float2* uniform_base(float2* p)
-{
-    const unsigned long long a = (unsigned long long)p;
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-    return (float2*)(((unsigned long long)hi << 32) | lo);
-}
-
-typedef unsigned StoreBits __attribute__((ext_vector_type(2)));
 
 // Every output index of stage S (no mirror) equals IndexSplit's U(t) + C(b, r) and lies in
 // [0, N): checked for every thread, butterfly and element at compile time.
@@ -329,7 +380,6 @@ __device__ __forceinline__ void last_stage_store(float2* v, int t,
     if constexpr (UNIFORM) {
         using IS = IndexSplit<PL, S, false, true>;
         static_assert(store_split_exact<PL, S>(), "store index split");
-        constexpr int kRawBufferFormat = 0x00020000;   // 32-bit data format, untyped access
         const __amdgpu_buffer_rsrc_t rs =
             __builtin_amdgcn_make_buffer_rsrc(uniform_base(dst), (short)0, PL::N * (int)sizeof(float2), kRawBufferFormat);
         const unsigned vo = (unsigned)IS::U(t, 0) * (unsigned)sizeof(float2);
@@ -503,7 +553,7 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
 #pragma unroll
             for (int b = 0; b < H; ++b) wl[b] = tw[2 * N + t + b * T];
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, true>(v, t, re, im);
-            last_stage_pack_mirror<PL, INV>(v, t, tw, tws, wl, dst, active);
+            last_stage_pack_mirror<PL, INV, SH::TPW == 1>(v, t, tw, tws, wl, dst, active);
         } else if constexpr (KIND == kRealForward) {
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
             stage_compute<PL, SL, INV>(v, t, tw);
