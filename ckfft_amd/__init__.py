@@ -26,7 +26,7 @@ __all__ = [
     "CkFftServerWait", "CkFftServerDrain", "CkFftServerGetStats", "CkFftServerResetStats",
     "CkFftServerDestroy", "CkFftServerLastError", "Server", "CkFftSetPrecision", "CkFftGetPrecision",
     "kCkFftPrecision_Fp32", "kCkFftPrecision_SplitBf16x3", "CkFftDeviceTableBytes", "CkFftInitArena",
-    "EXPORTED_SYMBOLS",
+    "CkFftTransformBatchNoAlloc", "NOALLOC_SCRATCH_BYTES", "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
@@ -63,6 +63,8 @@ kCkFftKind_RealInverse = 3
 kCkFftBatch_Graph = 1
 kCkFftPrecision_Fp32 = 0
 kCkFftPrecision_SplitBf16x3 = 1
+# CKFFT_NOALLOC_SCRATCH_BYTES: the fixed per-GPU scratch of CkFftTransformBatchNoAlloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class CkFftBatchDesc(ctypes.Structure):
@@ -126,6 +128,7 @@ _SIGS = {
     "CkFftDeviceTableBytes": (ctypes.c_size_t, [_i, _i]),
     "CkFftInitArena": (_vp, [_i, _i, _vp, ctypes.POINTER(ctypes.c_size_t), _vp, ctypes.c_size_t]),
     "CkFftGetPrecision": (_i, [_vp]),
+    "CkFftTransformBatchNoAlloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -218,6 +221,11 @@ class Context:
 
     def real_inverse_batch(self, n, count, y, x, stream=None) -> int:
         return CkFftRealInverseBatch(self.handle, n, count, _ptr(y), _ptr(x), _stream(stream))
+
+    def transform_batch_noalloc(self, kind, n, count, x, y, stream=None) -> int:
+        """CkFftTransformBatchNoAlloc: the batched call for any kind (kCkFftKind_*) with the
+        multi-pass intermediate in the library's fixed per-GPU scratch (no allocation)."""
+        return CkFftTransformBatchNoAlloc(self.handle, kind, n, count, _ptr(x), _ptr(y), _stream(stream))
 
     def twiddles(self, L: int, out) -> int:
         return CkFftGetTwiddles(self.handle, L, _ptr(out))

@@ -115,6 +115,41 @@ int run_batch(CkFftContext* c, Kind kind, int n, int count, const void* in, void
     return record(e, "launch") ? 1 : 0;
 }
 
+// CkFftTransformBatchNoAlloc: run_batch's rules, with the multi-pass intermediate in the
+// fixed per-device scratch array.  Nothing on this path allocates: validation reads
+// pointer attributes, and launch_transform is given the scratch.
+int run_batch_noalloc(CkFftContext* c, Kind kind, int n, int count, const void* in, void* out, void* stream)
+{
+    if (!size_ok(c, n) || count < 0) return 0;
+    if (!in || !out || in == out) return 0;
+    const int log2n = ilog2(n);
+    if (log2n > max_log2n(kind)) return 0;
+    if (count == 0) return 1;
+    if (!is_device_pointer(in) || !is_device_pointer(out)) return 0;
+    const DeviceTables tb = context_tables(c);
+    const size_t need = transform_scratch_bytes(kind, log2n, count, tb);
+    void* scratch = nullptr;
+    if (need > 0) {
+        if (need > CKFFT_NOALLOC_SCRATCH_BYTES) {
+            std::snprintf(g_last_error, sizeof(g_last_error),
+                          "no-alloc call: intermediate of %zu bytes exceeds the fixed scratch of %zu bytes", need,
+                          (size_t)CKFFT_NOALLOC_SCRATCH_BYTES);
+            return 0;
+        }
+        scratch = noalloc_scratch(c->device);
+        if (!scratch) {
+            std::snprintf(g_last_error, sizeof(g_last_error),
+                          "no-alloc call: the fixed scratch of device %d is unavailable (its address lookup at "
+                          "CkFftInit failed)",
+                          c->device);
+            return 0;
+        }
+    }
+    hipError_t e = launch_transform(kind, log2n, count, in, out, tb, (hipStream_t)stream, scratch,
+                                    scratch ? (size_t)CKFFT_NOALLOC_SCRATCH_BYTES : 0);
+    return record(e, "launch") ? 1 : 0;
+}
+
 // The context's device tables, in one block: the pyramid, the fine tables (levels above
 // 2^16) and the transposed inter-pass table, each 256-byte aligned.
 struct TableLayout {
@@ -187,6 +222,10 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
         release();
         return nullptr;
     }
+
+    // the no-allocation entry point's scratch array: its address on this (current) device,
+    // looked up once per device; a failure only disables that entry point
+    (void)record(noalloc_scratch_setup(dev), "no-alloc scratch lookup");
 
     void* mem = buf ? buf : std::malloc(need);
     if (!mem) {
@@ -337,6 +376,16 @@ int CkFftRealInverseBatch(CkFftContext* context, int n, int count, const CkFftCo
 {
     if (!context_ok(context, kCkFftDirection_Inverse)) return 0;
     return run_batch(context, kRealInverse, n, count, input, output, stream);
+}
+
+int CkFftTransformBatchNoAlloc(CkFftContext* context, int kind, int n, int count, const void* input, void* output,
+                               void* stream)
+{
+    if (kind < kCkFftKind_ComplexForward || kind > kCkFftKind_RealInverse) return 0;
+    const int dir = (kind == kCkFftKind_ComplexForward || kind == kCkFftKind_RealForward) ? kCkFftDirection_Forward
+                                                                                          : kCkFftDirection_Inverse;
+    if (!context_ok(context, dir)) return 0;
+    return run_batch_noalloc(context, (Kind)kind, n, count, input, output, stream);
 }
 
 int CkFftGetTwiddles(CkFftContext* context, int L, CkFftComplex* out)

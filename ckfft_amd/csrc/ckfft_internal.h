@@ -66,9 +66,21 @@ hipError_t launch_twiddle_transposed(float2* table, const float2* pyramid, int l
 
 // Batched transform of `count` signals of size n (n >= 1, power of two, n <= the
 // kind's limit).  Pointers are device memory with the layouts of ckfft_gpu.h.  Sizes
-// above one workgroup take a stream-ordered scratch allocation for the intermediate.
+// above one workgroup need an intermediate: `scratch` (device memory of scratch_bytes)
+// when it is given, else a stream-ordered allocation.
 hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, void* out,
-                            const DeviceTables& tables, hipStream_t stream);
+                            const DeviceTables& tables, hipStream_t stream, void* scratch = nullptr,
+                            size_t scratch_bytes = 0);
+
+// Bytes of intermediate launch_transform needs for this transform (0: none).
+size_t transform_scratch_bytes(Kind kind, int log2n, int count, const DeviceTables& tables);
+
+// The fixed per-device scratch array of CkFftTransformBatchNoAlloc (fft_pass.hip):
+// noalloc_scratch_setup looks up its address on `device`, which must be the current
+// device, once per device; noalloc_scratch returns that address (null before a successful
+// setup on that device).
+hipError_t noalloc_scratch_setup(int device);
+void* noalloc_scratch(int device);
 
 // Largest log2(n) launch_transform supports for a kind.
 int max_log2n(Kind kind);
@@ -89,8 +101,12 @@ hipError_t staged_transform(StagingPool* pool, Kind kind, int log2n, const void*
                             size_t out_bytes, const DeviceTables& tables);
 
 // Multi-pass path (fft_pass.hip) for sizes above one workgroup.
+// `scratch`: the intermediate (multipass_scratch_bytes), or null to allocate it on the
+// stream from the tables' pool.
 hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, void* out,
-                            const DeviceTables& tables, hipStream_t stream);
+                            const DeviceTables& tables, hipStream_t stream, void* scratch = nullptr,
+                            size_t scratch_bytes = 0);
+size_t multipass_scratch_bytes(Kind kind, int log2n, int count);
 
 }  // namespace ckfft_gpu
 

@@ -184,8 +184,23 @@ static hipError_t launch_tiny(Kind kind, int log2n, int count, const void* in, v
 
 int max_log2n(Kind) { return kLogMaxSupported; }
 
+// The route launch_transform takes; true when it is the multi-pass one (which needs an
+// intermediate).
+static bool takes_multipass(Kind kind, int log2n, const DeviceTables& tb)
+{
+    const bool real = (kind == kRealForward || kind == kRealInverse);
+    if (log2n <= (real ? 2 : 1)) return false;
+    if (tb.precision == kCkFftPrecision_SplitBf16x3 && mfma_supported(kind, log2n)) return false;
+    return (real ? log2n - 1 : log2n) > kLogMaxLds;
+}
+
+size_t transform_scratch_bytes(Kind kind, int log2n, int count, const DeviceTables& tb)
+{
+    return takes_multipass(kind, log2n, tb) ? multipass_scratch_bytes(kind, log2n, count) : 0;
+}
+
 hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, void* out, const DeviceTables& tb,
-                            hipStream_t stream)
+                            hipStream_t stream, void* scratch, size_t scratch_bytes)
 {
     if (log2n < 0 || log2n > max_log2n(kind) || log2n > tb.log_max || count < 0) return hipErrorInvalidValue;
     if (count == 0) return hipSuccess;
@@ -200,7 +215,7 @@ hipError_t launch_transform(Kind kind, int log2n, int count, const void* in, voi
         if (tb.log_pyramid < log2n) return hipErrorInvalidValue;
         return launch_lds(kind, logc, count, in, out, tb.pyramid, stream);
     }
-    return launch_multipass(kind, log2n, count, in, out, tb, stream);
+    return launch_multipass(kind, log2n, count, in, out, tb, stream, scratch, scratch_bytes);
 }
 
 }  // namespace ckfft_gpu

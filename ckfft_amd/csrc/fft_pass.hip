@@ -414,24 +414,73 @@ static hipError_t run_passes(Kind edge, bool inv, int logc, long long count, con
     return hipSuccess;
 }
 
+size_t multipass_scratch_bytes(Kind kind, int log2n, int count)
+{
+    const bool real = (kind == kRealForward || kind == kRealInverse);
+    const int logc = real ? log2n - 1 : log2n;
+    return sizeof(float2) * ((size_t)1 << logc) * (size_t)count;
+}
+
 hipError_t launch_multipass(Kind kind, int log2n, int count, const void* in, void* out, const DeviceTables& tb,
-                            hipStream_t s)
+                            hipStream_t s, void* scratch, size_t scratch_bytes)
 {
     const bool real = (kind == kRealForward || kind == kRealInverse);
     const int logc = real ? log2n - 1 : log2n;
     int logr[3];
     if (!plan_passes(logc, logr)) return hipErrorInvalidValue;
     if (log2n > tb.log_max) return hipErrorInvalidValue;
-    const long long pts = (1ll << logc) * (long long)count;
+    const size_t tmp_bytes = multipass_scratch_bytes(kind, log2n, count);
+    const bool inv = kind == kComplexInverse || kind == kRealInverse;
+    if (scratch) {
+        if (scratch_bytes < tmp_bytes) return hipErrorInvalidValue;
+        return run_passes(kind, inv, logc, count, (const float2*)in, (float2*)out, (float2*)scratch, tb, s);
+    }
     float2* tmp = nullptr;
-    const size_t tmp_bytes = sizeof(float2) * pts;
     hipError_t e = tb.pool ? hipMallocFromPoolAsync((void**)&tmp, tmp_bytes, tb.pool, s)
                            : hipMallocAsync((void**)&tmp, tmp_bytes, s);
     if (e != hipSuccess) return e;
-    e = run_passes(kind, kind == kComplexInverse || kind == kRealInverse, logc, count, (const float2*)in, (float2*)out,
-                   tmp, tb, s);
+    e = run_passes(kind, inv, logc, count, (const float2*)in, (float2*)out, tmp, tb, s);
     hipError_t e2 = hipFreeAsync(tmp, s);
     return e != hipSuccess ? e : e2;
+}
+
+// ---- the fixed scratch of the no-allocation entry point ---------------------------------
+//
+// CkFftTransformBatchNoAlloc takes its multi-pass intermediates from this array instead of
+// allocating them.  Every GPU the library runs on holds its own copy, at its own address,
+// from the code object's load until the process ends.  In host code `scratch` names a
+// host-side stand-in, never device memory: kernels get the address hipGetSymbolAddress
+// returned for the device (g_scratch_ptr).  The intermediate is written in full by one pass
+// before the next reads it, so nothing needs clearing between calls; calls that overlap on
+// one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[CKFFT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace ckfft_gpu

@@ -35,6 +35,23 @@ int CkFftRealForwardBatch(CkFftContext* context, int n, int count,
 int CkFftRealInverseBatch(CkFftContext* context, int n, int count,
                           const CkFftComplex* input, float* output, void* stream);
 
+/* ---- a batched call that allocates nothing ------------------------------------------
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  It is
+ * the batched call above for any kind (kCkFftKind_*, below), with the same arguments,
+ * validation and results (bit for bit), except that the sizes above one workgroup
+ * (complex n > 2^15, real n > 2^16) take their intermediate from a fixed device array the
+ * library holds on every GPU it runs on, instead of allocating it.  That array is
+ * CKFFT_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU, held until the process ends; a transform
+ * whose intermediate (8 n bytes per complex signal, 4 n per real signal, times count)
+ * exceeds it is rejected (0, and CkFftGpuLastError says why).  The call allocates nothing
+ * on any path.  Calls that overlap on one GPU share the array and race: serialise them
+ * (one stream, or events).  Returns 1 when the work was enqueued, 0 otherwise. */
+#define CKFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int CkFftTransformBatchNoAlloc(CkFftContext* context, int kind, int n, int count, const void* input, void* output,
+                               void* stream);
+
 /* ---- uneven batches in persistent launches --------------------------------------------
  *
  * A batch plan takes any mix of transforms -- each with its own kind, size and device
