@@ -63,40 +63,36 @@ def load_traffic(workload: str):
         return json.load(f)
 
 
-def cpu_baseline(n: int, seconds: float = 10.0):
-    """Reference CPU path on this host: all cores, bounded sample, GFLOP/s."""
-    import oracle as O
+def cpu_info():
+    """This host: CPU model, the cores this process may run on (sched_getaffinity), the
+    cgroup CPU quota if one is set, and the thread count the CPU baseline uses: every
+    affinity core, but no more than the CPU share (the cgroup quota, else the 16 host cores
+    a one-GPU box allots per GPU) -- threads beyond the share would only time-slice, or take
+    cores from other tenants of the node."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    share = math.ceil(quota) if quota else 16
+    return {"cpu": model, "affinity_cores": affinity, "cgroup_cpu_quota": quota, "share_cores": share,
+            "threads": max(1, min(affinity, share)), "online_cpus": os.cpu_count()}
 
-    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
-    sample = 64
-    x = O.normal01(0x5EED, sample * n).reshape(sample, n)
-    use_ref = O.Reference.available()
-    if use_ref:
-        R = O.Reference()
-        ctx = R.init(n, 3)
-        lib = R.lib
-        yb = np.empty((sample, 2 * (n // 2 + 1)), np.float32)
-        zb = np.empty((sample, n), np.float32)
-        tmps = [np.empty(2 * (n // 2 + 1), np.float32) for _ in range(threads)]
 
-        def work(t):
-            for b in range(t, sample, threads):
-                lib.CkFftRealForward(ctx, n, x[b].ctypes.data, yb[b].ctypes.data)
-                lib.CkFftRealInverse(ctx, n, yb[b].ctypes.data, zb[b].ctypes.data, tmps[t].ctypes.data)
-
-        def one_pass():
-            ts = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
-            for th in ts:
-                th.start()
-            for th in ts:
-                th.join()
-    else:
-        o = O.Restatement(n, 3)
-
-        def one_pass():
-            y = o.real_forward(x, threads)
-            o.real_inverse(y, n, threads)
-
+def _timed_passes(one_pass, seconds: float):
     one_pass()  # warm
     passes, t0 = 0, time.perf_counter()
     while True:
@@ -104,35 +100,165 @@ def cpu_baseline(n: int, seconds: float = 10.0):
         passes += 1
         el = time.perf_counter() - t0
         if el >= seconds:
-            break
-    # one core: one signal's forward + inverse at a time, median of 200 repetitions
+            return passes, el
+
+
+def _median_us(fn, reps: int) -> float:
+    t = []
+    for k in range(reps):
+        t1 = time.perf_counter()
+        fn(k)
+        t.append(time.perf_counter() - t1)
+    return float(np.median(t)) * 1e6
+
+
+def cpu_baseline(n: int, seconds: float = 10.0, info=None):
+    """Reference CPU path on this host for the benchmarked workload (real forward +
+    inverse, n = 65536): `threads` threads sharing one context on a bounded sample of
+    signals, and one core timing one signal's forward + inverse, median of 500 repetitions
+    (the reference's own convention, src/test/test.cpp:611).  GFLOP/s at 2.5 N log2 N per
+    real transform, with the literal 5 N log2 N figure beside it."""
+    import oracle as O
+
+    info = info or cpu_info()
+    threads = info["threads"]
+    sample = 64
+    x = O.normal01(0x5EED, sample * n).reshape(sample, n)
+    use_ref = O.Reference.available()
     single = None
     if use_ref:
-        reps = []
-        for k in range(200):
-            t1 = time.perf_counter()
+        R = O.Reference()
+        ctx = R.init(n, 3)
+        lib = R.lib
+        yb = np.empty((sample, 2 * (n // 2 + 1)), np.float32)
+        zb = np.empty((sample, n), np.float32)
+        tmp = np.empty(2 * (n // 2 + 1), np.float32)
+
+        def one_pass():
+            R.run_threads(ctx, [2] * sample, [n] * sample, list(x), list(yb), threads)
+            R.run_threads(ctx, [3] * sample, [n] * sample, list(yb), list(zb), threads)
+
+        passes, el = _timed_passes(one_pass, seconds)
+
+        def one(k):
             lib.CkFftRealForward(ctx, n, x[k % sample].ctypes.data, yb[0].ctypes.data)
-            lib.CkFftRealInverse(ctx, n, yb[0].ctypes.data, zb[0].ctypes.data, tmps[0].ctypes.data)
-            reps.append(time.perf_counter() - t1)
-        med = float(np.median(reps))
+            lib.CkFftRealInverse(ctx, n, yb[0].ctypes.data, zb[0].ctypes.data, tmp.ctypes.data)
+
+        one(0)
+        med = _median_us(one, 500) * 1e-6
         single = {"value": round(2 * real_flops(n) / med / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
-                  "us_per_forward_plus_inverse": round(med * 1e6, 1), "sample": "median of 200 repetitions"}
+                  "value_literal_5N": round(4 * real_flops(n) / med / 1e9, 3),
+                  "us_per_forward_plus_inverse": round(med * 1e6, 1), "sample": "median of 500 repetitions"}
         R.shutdown(ctx)
+    else:
+        o = O.Restatement(n, 3)
+
+        def one_pass():
+            y = o.real_forward(x, threads)
+            o.real_inverse(y, n, threads)
+
+        passes, el = _timed_passes(one_pass, seconds)
     gflops = passes * sample * 2 * real_flops(n) / el / 1e9
-    cpu = platform.processor() or platform.machine()
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads,
+            "value_literal_5N": round(2 * gflops, 3),
             "kind": "reference" if use_ref else "port",
             "sample": f"{passes} passes x {sample} signals x (real fwd + real inv), n={n}, "
-                      f"{el:.1f} s wall, {threads} threads sharing one context",
-            "cpu": cpu, "single_core": single}
+                      f"{el:.1f} s wall, {threads} POSIX threads sharing one context",
+            **{k: info[k] for k in ("cpu", "affinity_cores", "cgroup_cpu_quota", "share_cores", "online_cpus")},
+            "single_core": single}
+
+
+def cpu_other_configs(info, seconds: float = 4.0):
+    """The reference CPU path (oracle/_ref) beside the other workloads, on this host:
+      * complex 4096 x 1024 forward + inverse: `threads` threads over the 1024 signals
+        for `seconds`, and one core's median of 500 forward + inverse pairs;
+      * complex 2^24 forward: one transform on one core (the reference computes one
+        transform on one thread, so more cores do not shorten it), two repetitions;
+      * the uneven mix: a 2,000-transform sample of the same size / kind / direction
+        distribution, split over `threads` threads for `seconds`, and on one core.
+    GFLOP/s at 5 N log2 N complex and 2.5 N log2 N real."""
+    import oracle as O
+
+    if not O.Reference.available():
+        return None
+    R = O.Reference()
+    lib = R.lib
+    threads = info["threads"]
+    out = {}
+    # complex 4096 x 1024
+    n, count = 4096, 1024
+    ctx = R.init(n, 3)
+    x = O.uniform_pm1(0xC1, count * 2 * n).reshape(count, 2 * n)
+    y = np.empty_like(x)
+    z = np.empty_like(x)
+
+    def one_pass():
+        R.run_threads(ctx, [0] * count, [n] * count, list(x), list(y), threads)
+        R.run_threads(ctx, [1] * count, [n] * count, list(y), list(z), threads)
+
+    passes, el = _timed_passes(one_pass, seconds)
+
+    def one(k):
+        lib.CkFftComplexForward(ctx, n, x[k % count].ctypes.data, y[0].ctypes.data)
+        lib.CkFftComplexInverse(ctx, n, y[0].ctypes.data, z[0].ctypes.data)
+
+    med = _median_us(one, 500) * 1e-6
+    fl = 2 * 5.0 * n * 12
+    out["complex_4096_x1024"] = {
+        "value": round(passes * count * fl / el / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+        "sample": f"{passes} passes x {count} signals x (fwd + inv), {el:.1f} s wall",
+        "single_core": {"value": round(fl / med / 1e9, 3), "us_per_forward_plus_inverse": round(med * 1e6, 1),
+                        "sample": "median of 500 repetitions"}}
+    R.shutdown(ctx)
+    del x, y, z
+    # complex 2^24, one transform
+    n = 1 << 24
+    ctx = R.init(n, 1)
+    x = O.uniform_pm1(0xC3, 2 * n)
+    y = np.empty_like(x)
+    lib.CkFftComplexForward(ctx, n, x.ctypes.data, y.ctypes.data)
+    reps = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        lib.CkFftComplexForward(ctx, n, x.ctypes.data, y.ctypes.data)
+        reps.append(time.perf_counter() - t0)
+    t = min(reps)
+    out["complex_2p24"] = {"value": round(5.0 * n * 24 / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                           "ms": round(t * 1e3, 1), "sample": "one transform, best of 2 after one warm-up"}
+    R.shutdown(ctx)
+    del x, y
+    # uneven mix sample
+    rng = np.random.default_rng(0xC4)
+    m = 2000
+    ks = rng.integers(6, 17, size=m)
+    real = rng.random(m) < 0.5
+    fwd = rng.random(m) < 0.5
+    ctx = R.init(1 << 16, 3)
+    bufs = []
+    for i in range(m):
+        nn = 1 << int(ks[i])
+        if real[i] and fwd[i]:
+            a, b = O.normal01(i, nn), np.empty(2 * (nn // 2 + 1), np.float32)
+        elif real[i]:
+            a, b = O.normal01(i, 2 * (nn // 2 + 1)), np.empty(nn, np.float32)
+        else:
+            a, b = O.normal01(i, 2 * nn), np.empty(2 * nn, np.float32)
+        bufs.append((a, b))
+    flops = float(np.sum(np.where(real, 2.5, 5.0) * (1 << ks) * ks))
+
+    kinds = np.where(real, np.where(fwd, 2, 3), np.where(fwd, 0, 1))
+    ns = 1 << ks
+    ins = [b[0] for b in bufs]
+    outs = [b[1] for b in bufs]
+    passes, el = _timed_passes(lambda: R.run_threads(ctx, kinds, ns, ins, outs, threads), seconds)
+    t0 = time.perf_counter()
+    R.run_threads(ctx, kinds, ns, ins, outs, 1)
+    el1 = time.perf_counter() - t0
+    out["uneven_mix"] = {"value": round(passes * flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+                         "sample": f"{passes} passes x {m} transforms of the uneven distribution, {el:.1f} s wall",
+                         "single_core": {"value": round(flops / el1 / 1e9, 3), "sample": f"one pass of {m} transforms"}}
+    R.shutdown(ctx)
+    return out
 
 
 def uneven_mix(torch, dev, count: int, seed: int):
@@ -321,32 +447,64 @@ def back_to_back_ms(torch, stream, fn, reps: int, warmup: int = 2):
     return a.elapsed_time(b) / reps
 
 
-def complex_4096_leg(torch, C, O, dev, reps: int = 50):
+def complex_4096_leg(torch, C, O, dev, reps: int = 50, rotate_mib: int = 1024):
     """Complex forward then inverse round trip, N=4096, 1024 signals, re/im iid U[-1,1]
-    (SplitMix64 seed 0xC1): per-launch times (back to back; the median of individually
-    event-timed launches beside them), GB/s against 8 TB/s, errors of a sample."""
+    (SplitMix64 seed 0xC1): per-launch times back to back, cycling through buffer sets of
+    `rotate_mib` MiB in total (beyond the Infinity Cache), with the cache-resident times
+    (one buffer set) and the median of individually event-timed launches beside them;
+    GB/s against 8 TB/s; errors of a sample."""
     n, count = 4096, 1024
     x_h = O.uniform_pm1(0xC1, count * 2 * n).reshape(count, 2 * n)
-    x = torch.from_numpy(x_h).to(dev)
-    y = torch.empty_like(x)
-    z = torch.empty_like(x)
+    set_bytes = 3 * 8 * n * count
+    nsets = max(1, math.ceil((rotate_mib << 20) / set_bytes))
+    xs = [torch.from_numpy(x_h).to(dev)]
+    xs += [xs[0].clone() for _ in range(nsets - 1)]
+    ys = [torch.empty_like(xs[0]) for _ in range(nsets)]
+    zs = [torch.empty_like(xs[0]) for _ in range(nsets)]
     stream = torch.cuda.current_stream(dev)
+    reps = max(reps, 2 * nsets)
+    ctr = {"i": 0}
+
+    def nxt(rotate):
+        ctr["i"] += 1
+        return ctr["i"] % nsets if rotate else 0
+
     with C.Context(n, C.kCkFftDirection_Both) as ctx:
-        fwd = back_to_back_ms(torch, stream, lambda: ctx.complex_forward_batch(n, count, x, y, stream), reps)
-        inv = back_to_back_ms(torch, stream, lambda: ctx.complex_inverse_batch(n, count, y, z, stream), reps)
-        fwd_ev = event_ms(torch, stream, lambda: ctx.complex_forward_batch(n, count, x, y, stream), reps)
+        def fwd(rotate=True):
+            k = nxt(rotate)
+            return ctx.complex_forward_batch(n, count, xs[k], ys[k], stream)
+
+        def inv(rotate=True):
+            k = nxt(rotate)
+            return ctx.complex_inverse_batch(n, count, ys[k], zs[k], stream)
+
+        fwd_ms = back_to_back_ms(torch, stream, fwd, reps)
+        inv_ms = back_to_back_ms(torch, stream, inv, reps)
+        fwd_c = back_to_back_ms(torch, stream, lambda: fwd(False), reps)
+        inv_c = back_to_back_ms(torch, stream, lambda: inv(False), reps)
+        fwd_ev = event_ms(torch, stream, fwd, reps)
+        # the round trip on set 0 for the error sample
+        assert ctx.complex_forward_batch(n, count, xs[0], ys[0], stream) == 1
+        assert ctx.complex_inverse_batch(n, count, ys[0], zs[0], stream) == 1
     torch.cuda.synchronize()
     idx = [0, 511, 1023]
-    yh, zh = y[idx].cpu().numpy(), z[idx].cpu().numpy()
+    yh, zh = ys[0][idx].cpu().numpy(), zs[0][idx].cpu().numpy()
     o = O.Restatement(n, 3)
     nbytes = 16.0 * n * count
+    frac = lambda ms: round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    del xs, ys, zs
     return {"workload": "complex fwd + inv, N=4096 x 1024, U[-1,1]",
-            "forward_ms": round(fwd, 5), "inverse_ms": round(inv, 5),
-            "timing": "launches back to back between two events, elapsed / reps",
+            "forward_ms": round(fwd_ms, 5), "inverse_ms": round(inv_ms, 5),
+            "timing": f"{reps} launches back to back between two events, elapsed / {reps}, cycling through "
+                      f"{nsets} buffer sets ({nsets * set_bytes / 2**30:.2f} GiB)",
+            "working_set_bytes": nsets * set_bytes,
             "forward_ms_event_median": round(fwd_ev, 5),
-            "GFLOP_s": round(2 * 5.0 * n * 12 * count / ((fwd + inv) * 1e-3) / 1e9, 1),
-            "forward_GBs": round(nbytes / (fwd * 1e-3) / 1e9, 1),
-            "frac_of_hbm_peak": round(nbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "GFLOP_s": round(2 * 5.0 * n * 12 * count / ((fwd_ms + inv_ms) * 1e-3) / 1e9, 1),
+            "forward_GBs": round(nbytes / (fwd_ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": frac(fwd_ms), "inverse_frac_of_hbm_peak": frac(inv_ms),
+            "in_cache": {"forward_ms": round(fwd_c, 5), "inverse_ms": round(inv_c, 5),
+                         "frac_of_hbm_peak": frac(fwd_c), "inverse_frac_of_hbm_peak": frac(inv_c),
+                         "working_set_bytes": set_bytes},
             "forward_vs_ref": float(f"{O.split_norm_err(yh, o.complex(x_h[idx], False, 8)):.3e}"),
             "forward_relL2_fp64": float(f"{O.rel_l2(yh, O.interleave(O.fp64_complex(x_h[idx], False))):.3e}"),
             "round_trip_relL2": float(f"{O.rel_l2(zh / n, x_h[idx]):.3e}")}
@@ -472,6 +630,9 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16x3 precision leg")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the complex 4096 x 1024, 2^24 and host-pointer legs")
+    ap.add_argument("--rotate-mib", type=int, default=1024,
+                    help="rotate the timed launches through buffer sets of at least this many MiB in "
+                         "total (beyond the 256 MB Infinity Cache); 0 = one set")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 2 x 128 MiB (known bytes) to scale the "
                          "HBM counters FETCH_SIZE / WRITE_SIZE")
@@ -501,21 +662,28 @@ def main():
     n, B = N_REAL, BATCH
     half = n // 2 + 1
 
-    rng_x = None
     import oracle as O  # seeded input generator and the parity checker
     rng_x = O.normal01(0xBE7C4, B * n).reshape(B, n)
-    x = torch.from_numpy(rng_x).to(dev)
-    y = torch.empty((B, 2 * half), dtype=torch.float32, device=dev)
-    z = torch.empty((B, n), dtype=torch.float32, device=dev)
+    # Buffer sets (x, y, z) rotated from launch to launch, so that the working set of the
+    # timed launches (>= args.rotate_mib) exceeds the 256 MB Infinity Cache and every
+    # launch reads its input from HBM.  Every set holds the same input.
+    set_bytes = B * (4 * n + 8 * half + 4 * n)
+    nsets = max(1, math.ceil((args.rotate_mib << 20) / set_bytes))
+    xs = [torch.from_numpy(rng_x).to(dev)]
+    xs += [xs[0].clone() for _ in range(nsets - 1)]
+    ys = [torch.empty((B, 2 * half), dtype=torch.float32, device=dev) for _ in range(nsets)]
+    zs = [torch.empty((B, n), dtype=torch.float32, device=dev) for _ in range(nsets)]
+    x, y, z = xs[0], ys[0], zs[0]
     ctx = C.Context(n, C.kCkFftDirection_Both)
     stream = torch.cuda.current_stream(dev)
 
-    def step():
-        assert ctx.real_forward_batch(n, B, x, y, stream) == 1
-        assert ctx.real_inverse_batch(n, B, y, z, stream) == 1
+    def step(i=0):
+        k = i % nsets
+        assert ctx.real_forward_batch(n, B, xs[k], ys[k], stream) == 1
+        assert ctx.real_inverse_batch(n, B, ys[k], zs[k], stream) == 1
 
-    for _ in range(args.warmup):
-        step()
+    for i in range(max(args.warmup, nsets)):
+        step(i)
     torch.cuda.synchronize()
 
     # whole-step timing: K steps back to back between two events, synchronised on both
@@ -525,44 +693,50 @@ def main():
     torch.cuda.synchronize()
     t_start.record(stream)
     for k in range(args.steps):
-        step()
+        step(k)
     t_end.record(stream)
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
     ms_per_step = total_ms / args.steps
     # per-launch durations for the roofline: each kernel K times back to back between two
     # events on the launch stream, elapsed / K (the dispatch gaps between the launches are
-    # included, so these are upper bounds of the kernels' own durations)
-    def back_to_back(launch):
+    # included, so these are upper bounds of the kernels' own durations); `rotate` cycles
+    # through the buffer sets, else every launch reuses set 0 (cache-resident)
+    reps = max(args.steps, 2 * nsets)
+
+    def back_to_back(launch, rotate=True):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         a.record(stream)
-        for _ in range(args.steps):
-            launch()
+        for i in range(reps):
+            launch(i % nsets if rotate else 0)
         b.record(stream)
         torch.cuda.synchronize()
-        return a.elapsed_time(b) / args.steps
+        return a.elapsed_time(b) / reps
 
-    fwd_ms = back_to_back(lambda: ctx.real_forward_batch(n, B, x, y, stream))
-    inv_ms = back_to_back(lambda: ctx.real_inverse_batch(n, B, y, z, stream))
+    fwd_launch = lambda k: ctx.real_forward_batch(n, B, xs[k], ys[k], stream)
+    inv_launch = lambda k: ctx.real_inverse_batch(n, B, ys[k], zs[k], stream)
+    fwd_ms = back_to_back(fwd_launch)
+    inv_ms = back_to_back(inv_launch)
+    fwd_ms_cached = back_to_back(fwd_launch, rotate=False)
+    inv_ms_cached = back_to_back(inv_launch, rotate=False)
 
     # distribution: 100 launches of each kernel, each between its own pair of events (the
-    # events add a small gap to every launch), median and p99
+    # events add a small gap to every launch), median and p99; rotated buffer sets
     def distribution(launch, reps=100):
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
         torch.cuda.synchronize()
-        for a, b in ev:
+        for i, (a, b) in enumerate(ev):
             a.record(stream)
-            launch()
+            launch(i % nsets)
             b.record(stream)
         torch.cuda.synchronize()
         t = np.array([a.elapsed_time(b) for a, b in ev])
         return {"reps": reps, "median_ms": round(float(np.median(t)), 5), "p99_ms": round(float(np.percentile(t, 99)), 5)}
 
-    dist = {"real_forward": distribution(lambda: ctx.real_forward_batch(n, B, x, y, stream)),
-            "real_inverse": distribution(lambda: ctx.real_inverse_batch(n, B, y, z, stream)),
-            "timing": "each launch between its own pair of HIP events on the launch stream"}
+    dist = {"real_forward": distribution(fwd_launch), "real_inverse": distribution(inv_launch),
+            "timing": "each launch between its own pair of HIP events on the launch stream, buffer sets rotated"}
 
     flops_step = B * 2 * real_flops(n)
     value = flops_step / (ms_per_step * 1e-3) / 1e9
@@ -571,17 +745,24 @@ def main():
     achieved = bytes_launch / (dom_ms * 1e-3) / 1e9
     workload = f"real_fft_{n}_x{B}"
     traffic = load_traffic(workload)
+    frac = lambda ms: round(bytes_launch / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     roofline = {
         "bound": "hbm", "kernel": f"fft_lds_kernel ({dom_name.replace('_', ' ')}, n={n}: one workgroup per signal)",
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": (traffic or {}).get(dom_name),
         "algorithmic_bytes_per_launch": bytes_launch,
+        "working_set_bytes": nsets * set_bytes,
+        "buffer_sets": nsets,
         "launch_ms": {"real_forward": round(fwd_ms, 5), "real_inverse": round(inv_ms, 5)},
-        "launch_timing": "K launches of each kernel back to back between two events, elapsed / K",
+        "launch_timing": f"{reps} launches of each kernel back to back between two events, elapsed / {reps}, "
+                         f"cycling through {nsets} buffer sets ({nsets * set_bytes / 2**30:.2f} GiB)",
         "step_GBs": round(2 * bytes_launch / (ms_per_step * 1e-3) / 1e9, 1),
-        "frac_per_launch": {"real_forward": round(bytes_launch / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                            "real_inverse": round(bytes_launch / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "frac_per_launch": {"real_forward": frac(fwd_ms), "real_inverse": frac(inv_ms)},
+        "in_cache": {"launch_ms": {"real_forward": round(fwd_ms_cached, 5), "real_inverse": round(inv_ms_cached, 5)},
+                     "frac_per_launch": {"real_forward": frac(fwd_ms_cached), "real_inverse": frac(inv_ms_cached)},
+                     "working_set_bytes": set_bytes,
+                     "timing": "the same launches on one buffer set (Infinity Cache resident between launches)"},
         "launch_distribution": dist,
     }
 
@@ -616,25 +797,30 @@ def main():
 
     # host-pointer calls before the large legs below, on a small process footprint
     host = None if args.no_configs else host_pointer_leg(C, O)
-    cpu = None if args.no_cpu else cpu_baseline(n)
+    info = cpu_info()
+    cpu = None if args.no_cpu else cpu_baseline(n, info=info)
+    cpu_others = None if (args.no_cpu or args.no_configs) else cpu_other_configs(info)
     ctx.close()
-    del x, y, z
+    del x, y, z, xs, ys, zs
     uneven = None if args.no_uneven else uneven_leg(torch, C, O, dev, steps=10, warmup=10)
     serving = None if args.no_serve else serving_leg(torch, C, dev)
     split = None if args.no_split else split_bf16_leg(torch, C, O, dev)
     configs = None
     if not args.no_configs:
-        configs = {"complex_4096_x1024": complex_4096_leg(torch, C, O, dev),
+        configs = {"complex_4096_x1024": complex_4096_leg(torch, C, O, dev, rotate_mib=args.rotate_mib),
                    "complex_2p24": huge_leg(torch, C, O, dev),
-                   "host_pointers": host}
+                   "host_pointers": host,
+                   "cpu_baseline": cpu_others}
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "value_literal_5N": round(2 * value, 2),
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload, "n": n, "batch": B, "kinds": "real forward + real inverse",
-                   "flops_convention": "2.5*N*log2(N) per real transform",
+                   "flops_convention": "2.5*N*log2(N) per real transform (value_literal_5N: 5*N*log2(N))",
+                   "buffer_sets": nsets, "working_set_bytes": nsets * set_bytes,
                    "input": "iid N(0,1) float32, SplitMix64/Box-Muller seed 0xBE7C4"},
         "roofline": roofline,
         "cpu_baseline": cpu,

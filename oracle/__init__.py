@@ -147,6 +147,26 @@ class Reference:
     def available(so_path: str = REF_SO) -> bool:
         return os.path.exists(so_path)
 
+    def run_threads(self, ctx, kinds, ns, ins, outs, threads: int, passes: int = 1) -> None:
+        """`passes` passes over the transforms (kinds[i]: 0 complex forward, 1 complex
+        inverse, 2 real forward, 3 real inverse; ns[i]; numpy buffers ins[i] -> outs[i]),
+        transform i on POSIX thread i % threads, every thread sharing `ctx`
+        (oracle/ref_threads.c, built beside the reference library)."""
+        if not hasattr(self, "_thr"):
+            thr = ctypes.CDLL(os.path.join(os.path.dirname(REF_SO), "libckfft_ref_threads.so"))
+            thr.refthr_run.restype = ctypes.c_int
+            thr.refthr_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            self._thr = thr
+        count = len(ns)
+        k = np.ascontiguousarray(kinds, dtype=np.int32)
+        nn = np.ascontiguousarray(ns, dtype=np.int32)
+        pi = np.array([a.ctypes.data for a in ins], dtype=np.uint64)
+        po = np.array([a.ctypes.data for a in outs], dtype=np.uint64)
+        if not self._thr.refthr_run(ctx, count, k.ctypes.data, nn.ctypes.data, pi.ctypes.data, po.ctypes.data,
+                                    int(threads), int(passes)):
+            raise ValueError("reference rejected a transform, or a thread could not start")
+
     def init(self, max_n: int, direction: int = 3):
         return self.lib.CkFftInit(max_n, direction, None, None)
 
