@@ -86,6 +86,16 @@ __device__ __forceinline__ float2 w64(int q)
     return make_float2(kW64Re[q & 63], INV ? -kW64Im[q & 63] : kW64Im[q & 63]);
 }
 
+// exp(-2 pi i q / 128), q in [0, 128), rounded as kW64 (entry 2q equals kW64's entry q).
+__device__ constexpr float kW128Re[128] = {1.0f, 0.9987954497337341f, 0.9951847195625305f, 0.9891765117645264f, 0.9807852506637573f, 0.9700312614440918f, 0.9569403529167175f, 0.9415440559387207f, 0.9238795042037964f, 0.903989315032959f, 0.8819212913513184f, 0.8577286005020142f, 0.8314695954322815f, 0.803207516670227f, 0.7730104327201843f, 0.7409511208534241f, 0.7071067690849304f, 0.6715589761734009f, 0.6343932747840881f, 0.5956993103027344f, 0.5555702447891235f, 0.5141027569770813f, 0.4713967442512512f, 0.4275550842285156f, 0.3826834261417389f, 0.3368898630142212f, 0.290284663438797f, 0.24298018217086792f, 0.19509032368659973f, 0.1467304676771164f, 0.0980171412229538f, 0.049067676067352295f, 0.0f, -0.049067676067352295f, -0.0980171412229538f, -0.1467304676771164f, -0.19509032368659973f, -0.24298018217086792f, -0.290284663438797f, -0.3368898630142212f, -0.3826834261417389f, -0.4275550842285156f, -0.4713967442512512f, -0.5141027569770813f, -0.5555702447891235f, -0.5956993103027344f, -0.6343932747840881f, -0.6715589761734009f, -0.7071067690849304f, -0.7409511208534241f, -0.7730104327201843f, -0.803207516670227f, -0.8314695954322815f, -0.8577286005020142f, -0.8819212913513184f, -0.903989315032959f, -0.9238795042037964f, -0.9415440559387207f, -0.9569403529167175f, -0.9700312614440918f, -0.9807852506637573f, -0.9891765117645264f, -0.9951847195625305f, -0.9987954497337341f, -1.0f, -0.9987954497337341f, -0.9951847195625305f, -0.9891765117645264f, -0.9807852506637573f, -0.9700312614440918f, -0.9569403529167175f, -0.9415440559387207f, -0.9238795042037964f, -0.903989315032959f, -0.8819212913513184f, -0.8577286005020142f, -0.8314695954322815f, -0.803207516670227f, -0.7730104327201843f, -0.7409511208534241f, -0.7071067690849304f, -0.6715589761734009f, -0.6343932747840881f, -0.5956993103027344f, -0.5555702447891235f, -0.5141027569770813f, -0.4713967442512512f, -0.4275550842285156f, -0.3826834261417389f, -0.3368898630142212f, -0.290284663438797f, -0.24298018217086792f, -0.19509032368659973f, -0.1467304676771164f, -0.0980171412229538f, -0.049067676067352295f, 0.0f, 0.049067676067352295f, 0.0980171412229538f, 0.1467304676771164f, 0.19509032368659973f, 0.24298018217086792f, 0.290284663438797f, 0.3368898630142212f, 0.3826834261417389f, 0.4275550842285156f, 0.4713967442512512f, 0.5141027569770813f, 0.5555702447891235f, 0.5956993103027344f, 0.6343932747840881f, 0.6715589761734009f, 0.7071067690849304f, 0.7409511208534241f, 0.7730104327201843f, 0.803207516670227f, 0.8314695954322815f, 0.8577286005020142f, 0.8819212913513184f, 0.903989315032959f, 0.9238795042037964f, 0.9415440559387207f, 0.9569403529167175f, 0.9700312614440918f, 0.9807852506637573f, 0.9891765117645264f, 0.9951847195625305f, 0.9987954497337341f};
+__device__ constexpr float kW128Im[128] = {0.0f, -0.049067676067352295f, -0.0980171412229538f, -0.1467304676771164f, -0.19509032368659973f, -0.24298018217086792f, -0.290284663438797f, -0.3368898630142212f, -0.3826834261417389f, -0.4275550842285156f, -0.4713967442512512f, -0.5141027569770813f, -0.5555702447891235f, -0.5956993103027344f, -0.6343932747840881f, -0.6715589761734009f, -0.7071067690849304f, -0.7409511208534241f, -0.7730104327201843f, -0.803207516670227f, -0.8314695954322815f, -0.8577286005020142f, -0.8819212913513184f, -0.903989315032959f, -0.9238795042037964f, -0.9415440559387207f, -0.9569403529167175f, -0.9700312614440918f, -0.9807852506637573f, -0.9891765117645264f, -0.9951847195625305f, -0.9987954497337341f, -1.0f, -0.9987954497337341f, -0.9951847195625305f, -0.9891765117645264f, -0.9807852506637573f, -0.9700312614440918f, -0.9569403529167175f, -0.9415440559387207f, -0.9238795042037964f, -0.903989315032959f, -0.8819212913513184f, -0.8577286005020142f, -0.8314695954322815f, -0.803207516670227f, -0.7730104327201843f, -0.7409511208534241f, -0.7071067690849304f, -0.6715589761734009f, -0.6343932747840881f, -0.5956993103027344f, -0.5555702447891235f, -0.5141027569770813f, -0.4713967442512512f, -0.4275550842285156f, -0.3826834261417389f, -0.3368898630142212f, -0.290284663438797f, -0.24298018217086792f, -0.19509032368659973f, -0.1467304676771164f, -0.0980171412229538f, -0.049067676067352295f, 0.0f, 0.049067676067352295f, 0.0980171412229538f, 0.1467304676771164f, 0.19509032368659973f, 0.24298018217086792f, 0.290284663438797f, 0.3368898630142212f, 0.3826834261417389f, 0.4275550842285156f, 0.4713967442512512f, 0.5141027569770813f, 0.5555702447891235f, 0.5956993103027344f, 0.6343932747840881f, 0.6715589761734009f, 0.7071067690849304f, 0.7409511208534241f, 0.7730104327201843f, 0.803207516670227f, 0.8314695954322815f, 0.8577286005020142f, 0.8819212913513184f, 0.903989315032959f, 0.9238795042037964f, 0.9415440559387207f, 0.9569403529167175f, 0.9700312614440918f, 0.9807852506637573f, 0.9891765117645264f, 0.9951847195625305f, 0.9987954497337341f, 1.0f, 0.9987954497337341f, 0.9951847195625305f, 0.9891765117645264f, 0.9807852506637573f, 0.9700312614440918f, 0.9569403529167175f, 0.9415440559387207f, 0.9238795042037964f, 0.903989315032959f, 0.8819212913513184f, 0.8577286005020142f, 0.8314695954322815f, 0.803207516670227f, 0.7730104327201843f, 0.7409511208534241f, 0.7071067690849304f, 0.6715589761734009f, 0.6343932747840881f, 0.5956993103027344f, 0.5555702447891235f, 0.5141027569770813f, 0.4713967442512512f, 0.4275550842285156f, 0.3826834261417389f, 0.3368898630142212f, 0.290284663438797f, 0.24298018217086792f, 0.19509032368659973f, 0.1467304676771164f, 0.0980171412229538f, 0.049067676067352295f};
+
+template <bool INV = false>
+__device__ __forceinline__ float2 w128(int q)
+{
+    return make_float2(kW128Re[q & 127], INV ? -kW128Im[q & 127] : kW128Im[q & 127]);
+}
+
 // ---- in-register DFTs on a[0..R-1] (natural order in, natural order out) -------
 
 template <bool INV>
@@ -155,6 +165,56 @@ __device__ __forceinline__ void dft16(float2* a)
     t = a[11]; a[11] = a[14]; a[14] = t;
 }
 
+// a * W_32^M (forward: exp(-2 pi i M / 32)), M in [0, 16): the rotations of the radix-2
+// combine of dft32.  The exponents that are multiples of 2 are W_16 rotations.
+template <bool INV, int M>
+__device__ __forceinline__ float2 rot32(float2 a)
+{
+    if constexpr (M % 2 == 1)
+        return cmul(a, w128<INV>(4 * M));
+    else if constexpr (M / 2 == 5)              // W16^5 = W16^4 W16^1
+        return rot4<INV>(rot16<INV, 1>(a));
+    else if constexpr (M / 2 == 7)              // W16^7 = W16^4 W16^3
+        return rot4<INV>(rot16<INV, 3>(a));
+    else
+        return rot16<INV, M / 2>(a);
+}
+
+// 32-point DFT as a radix-2 step over two 16-point DFTs: X[k] = E[k] + W32^k O[k],
+// X[k+16] = E[k] - W32^k O[k], E/O the DFTs of the even/odd inputs.
+template <bool INV>
+__device__ __forceinline__ void dft32(float2* a)
+{
+    float2 e[16], o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        e[i] = a[2 * i];
+        o[i] = a[2 * i + 1];
+    }
+    dft16<INV>(e);
+    dft16<INV>(o);
+    o[1] = rot32<INV, 1>(o[1]);
+    o[2] = rot32<INV, 2>(o[2]);
+    o[3] = rot32<INV, 3>(o[3]);
+    o[4] = rot32<INV, 4>(o[4]);
+    o[5] = rot32<INV, 5>(o[5]);
+    o[6] = rot32<INV, 6>(o[6]);
+    o[7] = rot32<INV, 7>(o[7]);
+    o[8] = rot4<INV>(o[8]);
+    o[9] = rot32<INV, 9>(o[9]);
+    o[10] = rot32<INV, 10>(o[10]);
+    o[11] = rot32<INV, 11>(o[11]);
+    o[12] = rot32<INV, 12>(o[12]);
+    o[13] = rot32<INV, 13>(o[13]);
+    o[14] = rot32<INV, 14>(o[14]);
+    o[15] = rot32<INV, 15>(o[15]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        a[k] = cadd(e[k], o[k]);
+        a[k + 16] = csub(e[k], o[k]);
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* a)
 {
@@ -165,9 +225,11 @@ __device__ __forceinline__ void dft(float2* a)
         dft4<INV>(a[0], a[1], a[2], a[3]);
     } else if constexpr (R == 8) {
         dft8<INV>(a);
-    } else {
-        static_assert(R == 16, "radix must be 1, 2, 4, 8 or 16");
+    } else if constexpr (R == 16) {
         dft16<INV>(a);
+    } else {
+        static_assert(R == 32, "radix must be 1, 2, 4, 8, 16 or 32");
+        dft32<INV>(a);
     }
 }
 

@@ -41,18 +41,33 @@ namespace ckfft_gpu {
 // the radix plan, never at the thread layout: every layout of a size (any points per
 // thread >= 16) then has >= 2 butterflies per thread there and takes the same path, so
 // a transform rounds identically in every kernel that runs it.
+// Radix-32 edge stages occur only in the 2^15 plan, and every kernel runs 2^15 at 64
+// points per thread (two butterflies per thread), so the rule admits them too.
+constexpr bool mirror_radix(int rlog) { return rlog <= 3 || rlog == 5; }
+
 template <class PL, int KIND>
 constexpr int mirror_stage()
 {
     if constexpr (PL::T == 1) {
         return -1;
     } else if constexpr (KIND == kRealInverse) {
-        return (PL::rlog(0) <= 3 && bflies<PL, 0>() >= 2) ? 0 : -1;
+        return (mirror_radix(PL::rlog(0)) && bflies<PL, 0>() >= 2) ? 0 : -1;
     } else if constexpr (KIND == kRealForward) {
-        return (PL::rlog(PL::NSTAGES - 1) <= 3 && bflies<PL, PL::NSTAGES - 1>() >= 2) ? PL::NSTAGES - 1 : -1;
+        return (mirror_radix(PL::rlog(PL::NSTAGES - 1)) && bflies<PL, PL::NSTAGES - 1>() >= 2) ? PL::NSTAGES - 1
+                                                                                                 : -1;
     } else {
         return -1;
     }
+}
+
+// exp(-2 pi i (2(R-1-r)+1) / 4R): the pack/unpack twiddle of point M/2 + (R-1-r)*M of an
+// edge stage of radix R <= 32 (M = N/R), a constant (kW128 entry 2m equals kW64 entry m,
+// so the radices up to 16 get the values they had from kW64).
+template <int R>
+__device__ __forceinline__ float2 half_step_twiddle(int r)
+{
+    static_assert(R >= 1 && R <= 32, "edge radix");
+    return w128((2 * (R - 1 - r) + 1) * (32 / R));
 }
 
 // Stages S .. END-1, each after the exchange that brings its inputs.
@@ -133,6 +148,83 @@ __device__ __forceinline__ void load_first_unpack(float2* v, int t, const float2
     }
 }
 
+// Both unpacked values of a mirror pair (k, N-k): t[k] = unpack_one(Y[k], Y[N-k], w) and
+// t[N-k] = unpack_one(Y[N-k], Y[k], -conj w), sharing their sums, differences and product.
+// Bit-identical to the two unpack_one calls: the partner's sum and difference are the
+// negations (exact) of these, and cmul(conj(-conj w), (-d.x, d.y)) = (c.x, -c.y) exactly.
+__device__ __forceinline__ void unpack_pair(float2& yk, float2& ym, float2 w)
+{
+    const float2 a = make_float2(yk.x + ym.x, yk.y - ym.y);
+    const float2 d = make_float2(yk.x - ym.x, yk.y + ym.y);
+    const float2 c = cmul(cconj(w), d);
+    yk = make_float2(a.x - c.y, a.y + c.x);
+    ym = make_float2(a.x + c.y, c.x - a.y);
+}
+
+// Orders LDS accesses of different lanes of one wave: every access before it is complete
+// and visible to the wave's lanes after it (no barrier: the other waves do not take part).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The mirror-closed unpack for an edge radix above 8 (the radix-32 first stage of 2^15,
+// one butterfly pair per thread).  Every thread but 0 holds mirror pairs in its two
+// butterflies and unpacks them in place with unpack_pair.  Thread 0 holds the
+// self-mirrored butterflies 0 and M/2, whose partners are other elements of the same
+// butterflies: element (0, r), point r*M, pairs with element (0, R-r) (with Y[N] for
+// r = 0), and element (1, q), point M/2 + q*M, with element (1, R-1-q).  Thread 0 puts
+// their 2R original values in `part` (LDS outside the exchange planes), the 2R lanes of
+// its wave unpack one element each there, and thread 0 takes the results back after the
+// in-place pass, so no lane needs extra registers and no lane unpacks 2R points alone.
+template <class PL, bool INV>
+__device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t, const float2* __restrict__ y,
+                                                               const float2* __restrict__ tw, float* part)
+{
+    constexpr int N = PL::N;
+    constexpr int R = 1 << PL::rlog(0);
+    constexpr int B = PL::P / R;
+    constexpr int M = N / R;
+    static_assert(B == 2 && 2 * R == 64, "one mirror pair of butterflies per thread, 2R lanes in a wave");
+    const float2* lvl = tw + 2 * N;
+    const float2 wl = lvl[t];                           // W^t; W^(t + rM) = W^t exp(-2 pi i r/2R)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        v[r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, 0), r)];
+        v[R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, 1), r)];
+    }
+    float2* pp = reinterpret_cast<float2*>(part);
+    if (t < 2 * R) {                                    // thread 0's wave
+        if (t == 0) {
+#pragma unroll
+            for (int e = 0; e < 2 * R; ++e) pp[e] = v[e];
+        }
+        wave_lds_sync();
+        // lane t unpacks element t of thread 0's butterflies
+        const int q = t & (R - 1);
+        const float2 own = pp[t];
+        float2 res;
+        if (t < R) {
+            const float2 partner = t == 0 ? y[N] : pp[R - t];
+            res = unpack_one(own, partner, cmul(lvl[0], w64(q * (32 / R))));
+        } else {
+            res = unpack_one(own, pp[R + R - 1 - q], half_step_twiddle<R>(R - 1 - q));
+        }
+        wave_lds_sync();
+        pp[t] = res;
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) unpack_pair(v[r], v[R + R - 1 - r], cmul(wl, w64(r * (32 / R))));
+    if (t == 0) {
+#pragma unroll
+        for (int e = 0; e < 2 * R; ++e) v[e] = pp[e];
+    }
+    butterfly_compute<PL, 0, INV, true>(v, t, tw, 0);
+    butterfly_compute<PL, 0, INV, true>(v, t, tw, 1);
+}
+
 // Real inverse with a mirror-closed first stage, fused with that stage: each Y value is
 // read once, by the thread that also holds its mirror, and the pairs are unpacked in
 // registers.  Element (b, r) of a lower slot b < H is point k = jl + r*M and pairs with
@@ -143,13 +235,17 @@ __device__ __forceinline__ void load_first_unpack(float2* v, int t, const float2
 // butterflies computed.
 template <class PL, bool INV>
 __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, const float2* __restrict__ y,
-                                                          const float2* __restrict__ tw)
+                                                          const float2* __restrict__ tw, float* part)
 {
     constexpr int N = PL::N;
     constexpr int R = 1 << PL::rlog(0);
     constexpr int B = PL::P / R;
     constexpr int H = B / 2;
     constexpr int M = N / R;
+    if constexpr (R > 8) {
+        first_stage_unpack_mirror_wide<PL, INV>(v, t, y, tw, part);
+        return;
+    }
     const float2* lvl = tw + 2 * N;
     float2 q0[R], q1[R];                                // thread 0: partners of its first pair
     float2 wl[H];                                       // W^jl; W^(jl + rM) = W^jl exp(-2 pi i r/2R)
@@ -185,7 +281,7 @@ __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, cons
                 // W^(M/2 + (R-1-r)M) = exp(-2 pi i (2(R-1-r)+1) / 4R)
                 p0 = q0[r];
                 p1 = q1[r];
-                w1 = w64((2 * (R - 1 - r) + 1) * (16 / R));
+                w1 = half_step_twiddle<R>(r);
             }
             const float2 y0 = lo, y1 = hi;
             lo = unpack_one(y0, p0, w0);
@@ -281,11 +377,24 @@ constexpr bool pack_split_exact()
 // UNIFORM (one transform per workgroup): the stores go through a buffer resource over the
 // N+1 outputs, per-thread byte offsets in VGPRs and the compile-time parts as scalar
 // offsets (pack_split_exact), so no store needs 64-bit address arithmetic.
+// Both outputs of a mirror pair (k, N-k) of the real forward: out[k] = pack_one(Z[k],
+// Z[N-k], w), out[N-k] = pack_one(Z[N-k], Z[k], -conj w), sharing their sums, differences
+// and product (bit-identical to the two pack_one calls, as for unpack_pair).
+__device__ __forceinline__ void pack_pair_cse(float2 zk, float2 zm, float2 w, float2& ok, float2& om)
+{
+    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);
+    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+    const float2 c = cmul(w, d);
+    ok = make_float2(a.x + c.y, a.y - c.x);
+    om = make_float2(a.x - c.y, -(a.y + c.x));
+}
+
 template <class PL, bool INV, bool UNIFORM>
 __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const float2* __restrict__ tw,
                                                        StageTwiddles<PL, PL::NSTAGES - 1, INV, true>& tws,
                                                        float2 (&wl)[PL::P / (1 << PL::rlog(PL::NSTAGES - 1)) / 2],
-                                                       float2* __restrict__ dst, bool active)
+                                                       float2* __restrict__ dst, bool active, float* part,
+                                                       float* stage)
 {
     constexpr int S = PL::NSTAGES - 1;
     constexpr int N = PL::N;
@@ -295,6 +404,96 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
     constexpr int M = N / R;                    // the last stage writes point j + r*M
     const float2* lvl = tw + 2 * N;
     const int jh0 = bfly<PL, S, true>(t, H);    // M - t, or M/2 for t == 0
+    if constexpr (R > 8) {
+        // One mirror pair of butterflies per thread (the radix-32 last stage of 2^15).
+        // Every thread but 0 packs its pairs with pack_pair_cse; thread 0's butterflies 0
+        // and M/2 pair within themselves (element (0, q), point q*M, with element (0, R-q);
+        // element (1, q), point M/2 + q*M, with element (1, R-1-q)), so thread 0 puts their
+        // 2R values in `part` and the 2R lanes of its wave pack one point each.
+        // A row of N + 1 outputs that does not start on a 128-byte line (batched rows
+        // follow each other with a stride of N + 1 values) goes out through LDS in two
+        // halves, [0, N/2) and [N/2, N], each stored with every wave writing whole lines:
+        // stores that cover lines only partly cost far more HBM time than whole-line
+        // stores (complex 2^15 x 256 measured 36 us with its output on a line boundary,
+        // 45 us with it 8 bytes off).  A row on a line boundary is stored directly.
+        static_assert(H == 1 && 2 * R == 64, "one mirror pair of butterflies per thread, 2R lanes in a wave");
+        tws.settle();
+        settle_reads(wl);
+        butterfly_compute<PL, S, INV, true>(v, tws, 0);
+        butterfly_compute<PL, S, INV, true>(v, tws, 1);
+        float2 sp = make_float2(0.0f, 0.0f), nyq = make_float2(0.0f, 0.0f);
+        int spk = 0;                              // the point lane t of thread 0's wave packs
+        if (t < 2 * R) {
+            float2* pp = reinterpret_cast<float2*>(part);
+            if (t == 0) {
+#pragma unroll
+                for (int e = 0; e < 2 * R; ++e) pp[e] = v[e];
+                nyq = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
+            }
+            wave_lds_sync();
+            const int q = t & (R - 1);
+            if (t < R) {
+                spk = q * M;
+                sp = pack_one(pp[q], pp[(R - q) & (R - 1)], cmul(lvl[0], w64(q * (32 / R))));
+            } else {
+                spk = M / 2 + q * M;
+                sp = pack_one(pp[R + q], pp[R + R - 1 - q], half_step_twiddle<R>(R - 1 - q));
+            }
+        }
+        // in place: v[r] = out[t + rM], v[2R-1-r] = out[(M - t) + (R-1-r)M]
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 o0, o1;
+            pack_pair_cse(v[r], v[R + R - 1 - r], cmul(wl[0], w64(r * (32 / R))), o0, o1);
+            v[r] = o0;
+            v[R + R - 1 - r] = o1;
+        }
+        if (((unsigned long long)dst & 127u) == 0) {
+            // a row that starts on a line: the direct stores already cover whole lines
+            if (t != 0 && active) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    dst[t + r * M] = v[r];
+                    dst[(M - t) + (R - 1 - r) * M] = v[R + R - 1 - r];
+                }
+            }
+            if (t < 2 * R && active) dst[spk] = sp;
+            if (t == 0 && active) dst[N] = nyq;
+            return;
+        }
+        float2* stg = reinterpret_cast<float2*>(stage);   // the exchange planes are free now
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int c0 = half ? N / 2 : 0;
+            const int c1 = half ? N + 1 : N / 2;
+            // point t + rM lies in the upper half for r >= R/2; its mirror, for r < R/2
+            if (t != 0 && active) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if ((r >= R / 2) == (half == 1)) stg[t + r * M - c0] = v[r];
+                    else stg[(M - t) + (R - 1 - r) * M - c0] = v[R + R - 1 - r];
+                }
+            }
+            if (t < 2 * R && active && (spk >= N / 2) == (half == 1)) stg[spk - c0] = sp;
+            if (half == 1 && t == 0 && active) stg[N - c0] = nyq;
+            __syncthreads();
+            if (active) {
+                // points [c0, a) share a line with the part before them; from a on, every
+                // 64 points a wave stores are 4 whole lines
+                const unsigned mis = (unsigned)(((unsigned long long)(dst + c0)) & 127u);
+                const int a = c0 + (mis ? (int)((128u - mis) >> 3) : 0);
+                if (t < a - c0) dst[c0 + t] = stg[t];
+                const int cnt = c1 - a;
+#pragma unroll
+                for (int j = 0; j < (N / 2 + 1 + PL::T - 1) / PL::T; ++j) {
+                    const int i = t + j * PL::T;
+                    if (i < cnt) dst[a + i] = stg[a - c0 + i];
+                }
+            }
+            if (half == 0) __syncthreads();
+        }
+        return;
+    }
     __amdgpu_buffer_rsrc_t rs;
     unsigned vo0 = 0, voh0 = 0, vom = 0;
     if constexpr (UNIFORM) {
@@ -328,7 +527,7 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
                 const bool z = (t == 0);
                 p0 = z ? v[(R - r) & (R - 1)] : p0;
                 p1 = z ? v[H * R + r] : p1;
-                w1 = z ? w64((2 * (R - 1 - r) + 1) * (16 / R)) : w1;
+                w1 = z ? half_step_twiddle<R>(r) : w1;
             }
             if constexpr (UNIFORM) {
                 const int c0 = b * PL::T + r * M;
@@ -481,8 +680,18 @@ struct LdsShape {
     static constexpr int SPAN = plan_span<PL>();
     // LDS holds the exchanges (any plan of more than one stage, which includes every plan
     // of more than one thread)
-    static constexpr int LDS_WORDS = PL::NSTAGES > 1 ? (SPLIT ? SPAN : 2 * SPAN * TPW) : 1;
     static constexpr int MS = mirror_stage<PL, KIND>();
+    // a mirror-closed unpack of edge radix above 8 keeps thread 0's 2R unpacked values in
+    // LDS after the exchange planes (one transform per workgroup)
+    // a mirror-closed pack or unpack of edge radix above 8 passes thread 0's 2R points
+    // through LDS after the exchange planes (one transform per workgroup)
+    static constexpr int EDGE_LOG = MS < 0 ? 0 : PL::rlog(MS);
+    static constexpr bool WIDE_EDGE = MS >= 0 && EDGE_LOG > 3;
+    static_assert(!WIDE_EDGE || TPW == 1, "the wide pack/unpack keeps one transform's values");
+    static constexpr int PART_OFFSET = SPLIT ? SPAN : 2 * SPAN * TPW;
+    static constexpr int LDS_WORDS = PL::NSTAGES > 1 ? PART_OFFSET + (WIDE_EDGE ? 4 << EDGE_LOG : 0) : 1;
+    // the wide pack stages each half of the N+1 outputs (N + 2 floats) below PART_OFFSET
+    static_assert(!(WIDE_EDGE && KIND == kRealForward) || N + 2 <= PART_OFFSET, "pack staging fits the planes");
     // complex elements per signal on each side
     static constexpr int IN_STRIDE = KIND == kRealInverse ? N + 1 : N;
     static constexpr int OUT_STRIDE = KIND == kRealForward ? N + 1 : N;
@@ -534,7 +743,7 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
         constexpr int SL = PL::NSTAGES - 1;
         // first stage, fused with the loads (and the real-inverse unpack)
         if constexpr (KIND == kRealInverse && MS == 0) {
-            first_stage_unpack_mirror<PL, INV>(v, t, src, tw);
+            first_stage_unpack_mirror<PL, INV>(v, t, src, tw, lds + SH::PART_OFFSET);
         } else if constexpr (KIND == kRealInverse) {
             load_first_unpack<PL>(v, t, src, tw);
             stage_compute<PL, 0, INV>(v, t, tw);
@@ -553,7 +762,8 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
 #pragma unroll
             for (int b = 0; b < H; ++b) wl[b] = tw[2 * N + t + b * T];
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, true>(v, t, re, im);
-            last_stage_pack_mirror<PL, INV, SH::TPW == 1>(v, t, tw, tws, wl, dst, active);
+            last_stage_pack_mirror<PL, INV, SH::TPW == 1>(v, t, tw, tws, wl, dst, active, lds + SH::PART_OFFSET,
+                                                          lds);
         } else if constexpr (KIND == kRealForward) {
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
             stage_compute<PL, SL, INV>(v, t, tw);

@@ -13,13 +13,15 @@
 namespace ckfft_gpu {
 
 // Radix plan: every stage is radix 2^RMAX_LOG except one smaller "odd" stage, placed
-// first (SMALL_LAST = false) or last (SMALL_LAST = true).
+// first (SMALL_LAST = false) or last (SMALL_LAST = true).  The radix is 16, except for
+// 2^15 points at 64 per thread: three radix-32 stages (two butterflies per thread), so
+// two LDS exchanges instead of the three of 16 x 16 x 16 x 8.
 template <int LOGN, int LOGP, bool SMALL_LAST = false>
 struct Plan {
     static constexpr int N = 1 << LOGN;
     static constexpr int P = 1 << LOGP;          // points per thread
     static constexpr int T = N / P;              // threads per transform
-    static constexpr int RMAX_LOG = LOGP < 4 ? LOGP : 4;
+    static constexpr int RMAX_LOG = (LOGN == 15 && LOGP == 6) ? 5 : (LOGP < 4 ? LOGP : 4);
     static constexpr int NSTAGES = (LOGN + RMAX_LOG - 1) / RMAX_LOG;
     static constexpr int SMALL_LOG = LOGN - (NSTAGES - 1) * RMAX_LOG;
     static constexpr int rlog(int s)
@@ -101,7 +103,7 @@ __device__ __forceinline__ int bfly(int t, int b)
 // issue its reads early.
 template <int R>
 struct TwBase {
-    float2 w[R >= 16 ? 4 : R >= 8 ? 3 : R >= 4 ? 2 : 1];   // w, w^2, w^4, w^8
+    float2 w[R >= 32 ? 5 : R >= 16 ? 4 : R >= 8 ? 3 : R >= 4 ? 2 : 1];   // w, w^2, w^4, w^8, w^16
 };
 
 template <int R, bool INV>
@@ -112,13 +114,61 @@ __device__ __forceinline__ TwBase<R> twiddle_load(const float2* __restrict__ lvl
     if constexpr (R >= 4) b.w[1] = dirw<INV>(lvl[2 * k]);
     if constexpr (R >= 8) b.w[2] = dirw<INV>(lvl[4 * k]);
     if constexpr (R >= 16) b.w[3] = dirw<INV>(lvl[8 * k]);
+    if constexpr (R >= 32) b.w[4] = dirw<INV>(lvl[16 * k]);
     return b;
+}
+
+// The powers w^0 .. w^(R-1) from the power-of-two ones in tb, formed as twiddle_apply
+// forms them (w^0 = 1 exactly).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2* p, const TwBase<R>& tb)
+{
+    p[0] = make_float2(1.0f, 0.0f);
+    if constexpr (R >= 2) p[1] = tb.w[0];
+    if constexpr (R >= 4) {
+        p[2] = tb.w[1];
+        p[3] = cmul(p[1], p[2]);
+    }
+    if constexpr (R >= 8) {
+        p[4] = tb.w[2];
+        p[5] = cmul(p[1], p[4]);
+        p[6] = cmul(p[2], p[4]);
+        p[7] = cmul(p[3], p[4]);
+    }
+    if constexpr (R >= 16) {
+        p[8] = tb.w[3];
+        p[9] = cmul(p[1], p[8]);
+        p[10] = cmul(p[2], p[8]);
+        p[11] = cmul(p[3], p[8]);
+        p[12] = cmul(p[4], p[8]);
+        p[13] = cmul(p[1], p[12]);
+        p[14] = cmul(p[2], p[12]);
+        p[15] = cmul(p[3], p[12]);
+    }
+    if constexpr (R == 32) {
+        p[16] = tb.w[4];
+#pragma unroll
+        for (int m = 1; m < 16; ++m) p[16 + m] = cmul(p[16], p[m]);
+    }
 }
 
 template <int R>
 __device__ __forceinline__ void twiddle_apply(float2* a, const TwBase<R>& tb)
 {
-    if constexpr (R >= 2) {
+    if constexpr (R == 32) {
+        // w^0..w^15 as twiddle_apply<16> forms them, w^(16+m) = w^16 w^m
+        float2 p[16];
+        TwBase<16> lo;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lo.w[i] = tb.w[i];
+        twiddle_powers<16>(p, lo);
+        const float2 w16 = tb.w[4];
+#pragma unroll
+        for (int m = 1; m < 16; ++m) a[m] = cmul(a[m], p[m]);
+        a[16] = cmul(a[16], w16);
+#pragma unroll
+        for (int m = 1; m < 16; ++m) a[16 + m] = cmul(a[16 + m], cmul(w16, p[m]));
+    } else if constexpr (R >= 2) {
         const float2 w1 = tb.w[0];
         if constexpr (R == 2) {
             a[1] = cmul(a[1], w1);
@@ -149,35 +199,6 @@ __device__ __forceinline__ void twiddle_apply(float2* a, const TwBase<R>& tb)
                 }
             }
         }
-    }
-}
-
-// The powers w^0 .. w^(R-1) from the power-of-two ones in tb, formed as twiddle_apply
-// forms them (w^0 = 1 exactly).
-template <int R>
-__device__ __forceinline__ void twiddle_powers(float2* p, const TwBase<R>& tb)
-{
-    p[0] = make_float2(1.0f, 0.0f);
-    if constexpr (R >= 2) p[1] = tb.w[0];
-    if constexpr (R >= 4) {
-        p[2] = tb.w[1];
-        p[3] = cmul(p[1], p[2]);
-    }
-    if constexpr (R >= 8) {
-        p[4] = tb.w[2];
-        p[5] = cmul(p[1], p[4]);
-        p[6] = cmul(p[2], p[4]);
-        p[7] = cmul(p[3], p[4]);
-    }
-    if constexpr (R == 16) {
-        p[8] = tb.w[3];
-        p[9] = cmul(p[1], p[8]);
-        p[10] = cmul(p[2], p[8]);
-        p[11] = cmul(p[3], p[8]);
-        p[12] = cmul(p[4], p[8]);
-        p[13] = cmul(p[1], p[12]);
-        p[14] = cmul(p[2], p[12]);
-        p[15] = cmul(p[3], p[12]);
     }
 }
 
