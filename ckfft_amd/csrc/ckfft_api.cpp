@@ -92,7 +92,7 @@ int fail(hipError_t e, const char* where)
 }
 
 // One synchronous transform, as every drop-in call is: device buffers in place on the
-// null stream, host buffers through a staging slot.
+// calling thread's default stream, host buffers through a staging slot.
 int run_plain(CkFftContext* c, Kind kind, int n, const void* in, void* out)
 {
     const int log2n = ilog2(n);

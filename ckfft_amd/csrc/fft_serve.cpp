@@ -26,6 +26,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "ckfft/ckfft_gpu.h"
@@ -100,6 +101,9 @@ struct _CkFftServer {
     size_t lat_next = 0;
     long long completed = 0, failed = 0, batches = 0, batched = 0, done_bytes = 0;
     uint64_t t_first = 0, t_last = 0;
+    // tickets of failed requests, kept past the reuse of their ring entries so that a query
+    // of an old ticket still reports its failure (failures are rare: a launch error)
+    std::unordered_set<long long> failed_tickets;
 };
 
 namespace {
@@ -131,6 +135,7 @@ void finish_slot(CkFftServer* S, Slot& sl, bool ok)
                 S->done_bytes += r.bytes;
             } else {
                 ++S->failed;
+                S->failed_tickets.insert(id);
             }
             r.state = ok ? kDone : kFailed;
         }
@@ -184,7 +189,14 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
         ++S->batches;
         S->batched += (long long)sl.reqs.size();
     }
-    if (!ok) finish_slot(S, sl, false);
+    if (!ok) {
+        // work enqueued before the failing step may still be running on the slot's stream:
+        // wait for it before the requests are reported failed and their budget returned,
+        // so that no client reuses a buffer the device still writes
+        (void)hipStreamSynchronize(sl.stream);
+        (void)hipGetLastError();
+        finish_slot(S, sl, false);
+    }
     return ok;
 }
 
@@ -381,7 +393,8 @@ int CkFftServerQuery(CkFftServer* server, long long ticket)
     std::lock_guard<std::mutex> lock(server->m);
     if (ticket >= server->next_ticket) return -1;
     const Request& r = server->ring[ticket % kRing];
-    if (r.ticket != ticket) return 1;          // older than the ring: finished long ago
+    if (r.ticket != ticket)                    // older than the ring: finished long ago
+        return server->failed_tickets.count(ticket) ? -1 : 1;
     return r.state == kDone ? 1 : (r.state == kFailed ? -1 : 0);
 }
 
@@ -392,7 +405,8 @@ int CkFftServerWait(CkFftServer* server, long long ticket)
     if (ticket >= server->next_ticket) return 0;
     const Request& r = server->ring[ticket % kRing];
     server->cv.wait(lock, [&] { return r.ticket != ticket || r.state >= kDone; });
-    return r.ticket != ticket || r.state == kDone ? 1 : 0;
+    if (r.ticket != ticket) return server->failed_tickets.count(ticket) ? 0 : 1;
+    return r.state == kDone ? 1 : 0;
 }
 
 int CkFftServerDrain(CkFftServer* server)

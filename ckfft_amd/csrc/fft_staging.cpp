@@ -33,6 +33,10 @@ namespace ckfft_gpu {
 namespace {
 
 constexpr int kMaxSlots = 8;            // concurrent host-pointer calls per context
+// A slot keeps its buffers between calls up to this size; a call that needed more frees
+// them when it returns the slot, so a context does not hold the largest call's staging
+// memory (up to 8 x 2 x 4 GiB at 2^28) until CkFftShutdown.
+constexpr size_t kKeepBytes = (size_t)256 << 20;
 constexpr size_t kMinChunk = 1 << 20;   // bytes per pipelined copy chunk (at least)
 constexpr int kMaxChunks = 16;
 
@@ -128,6 +132,14 @@ Slot* acquire(StagingPool* p, hipError_t* err)
 
 void release(StagingPool* p, Slot* s)
 {
+    if (s->bytes > kKeepBytes) {
+        // the call synchronised its stream before returning the slot
+        (void)hipFree(s->d_buf);
+        (void)hipHostFree(s->h_buf);
+        s->d_buf = nullptr;
+        s->h_buf = nullptr;
+        s->bytes = 0;
+    }
     {
         std::lock_guard<std::mutex> lock(p->m);
         p->idle.push_back(s);
@@ -227,8 +239,11 @@ hipError_t staged_transform(StagingPool* pool, Kind kind, int log2n, const void*
 {
     const MemKind kin = classify(in), kout = classify(out);
     if (kin == kDevice && kout == kDevice) {
-        hipError_t e = launch_transform(kind, log2n, 1, in, out, tb, nullptr);
-        return e != hipSuccess ? e : hipStreamSynchronize(nullptr);
+        // the calling thread's own default stream: calls from several threads sharing a
+        // context (inc/ckfft/ckfft.h:39-41) overlap on the device instead of queueing on
+        // the null stream, which they still follow as a blocking stream does
+        hipError_t e = launch_transform(kind, log2n, 1, in, out, tb, hipStreamPerThread);
+        return e != hipSuccess ? e : hipStreamSynchronize(hipStreamPerThread);
     }
     hipError_t e = hipSuccess;
     Slot* s = acquire(pool, &e);

@@ -162,7 +162,7 @@ long long CkFftServerSubmit(CkFftServer* server, int kind, int n, const void* in
 int CkFftServerReplay(CkFftServer* server, const CkFftBatchDesc* descs, const double* arrivalUs, int count,
                       long long* tickets);
 /* 1 done, 0 pending, -1 failed or unknown ticket.  A ticket more than 2^18 submissions
- * old has finished and reports 1 (its failure, if any, is counted in the stats). */
+ * old has finished: it reports -1 if it failed, else 1 (failed tickets are remembered). */
 int CkFftServerQuery(CkFftServer* server, long long ticket);
 /* Block until the ticket completes; 1 on success. */
 int CkFftServerWait(CkFftServer* server, long long ticket);

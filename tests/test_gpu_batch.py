@@ -248,3 +248,49 @@ def test_config_uneven_sample_2000(torch, C, ctx):
         kind, n = mix[i]
         want = plain(torch, ctx, kind, n, ins[i]).cpu().numpy()
         assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
+
+
+def test_config_uneven_full_20000(torch, C, ctx):
+    """The whole uneven workload: 20,000 transforms, n = 2^k with k uniform in 6..16,
+    kinds and directions 50/50, N(0,1) data, each with its own 256-byte aligned buffers,
+    in one graph-replayed plan (run twice).  A bitwise sample of every 50th transform
+    against the single calls, and every 1,000th against the CPU oracle."""
+    count = 20000
+    rng = np.random.default_rng(0xC4)
+    ks = rng.integers(6, 17, size=count)
+    real = rng.random(count) < 0.5
+    fwd = rng.random(count) < 0.5
+    kinds = np.where(real, np.where(fwd, 2, 3), np.where(fwd, 0, 1))
+    ns = (1 << ks).astype(np.int64)
+    il = np.where(kinds == 2, ns, np.where(kinds == 3, 2 * (ns // 2 + 1), 2 * ns))
+    ol = np.where(kinds == 2, 2 * (ns // 2 + 1), np.where(kinds == 3, ns, 2 * ns))
+    ioff = np.concatenate([[0], np.cumsum((il + 63) // 64 * 64)])
+    ooff = np.concatenate([[0], np.cumsum((ol + 63) // 64 * 64)])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xC4)
+    x = torch.randn(int(ioff[-1]), device="cuda", generator=gen)
+    y = torch.full((int(ooff[-1]),), float("nan"), device="cuda")
+    xs = [x[int(ioff[i]): int(ioff[i]) + int(il[i])] for i in range(count)]
+    ys = [y[int(ooff[i]): int(ooff[i]) + int(ol[i])] for i in range(count)]
+    descs = [(int(kinds[i]), int(ns[i]), xs[i], ys[i]) for i in range(count)]
+    print("uneven 20000: building plan", flush=True)
+    plan = C.BatchPlan(ctx, descs, graph=True)
+    info = plan.info()
+    assert info["transforms"] == count and info["extra_launches"] == 0
+    assert info["persistent_transforms"] + info["pass_transforms"] == count
+    for _ in range(2):
+        assert plan.execute() == 1
+    torch.cuda.synchronize()
+    # every output value written: the only NaNs left are the alignment gaps between rows
+    assert int(torch.isnan(y).sum()) == int(ooff[-1]) - int(ol.sum())
+    for i in range(0, count, 50):
+        want = plain(torch, ctx, KINDS[kinds[i]], int(ns[i]), xs[i]).cpu().numpy()
+        assert np.array_equal(ys[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kinds[i], ns[i])
+    for i in range(0, count, 1000):
+        kind, n = KINDS[kinds[i]], int(ns[i])
+        xi = xs[i].cpu().numpy()[None]
+        o = O.Restatement(n, 3)
+        ref = (o.complex(xi, kind == "cinv") if kind in ("cfwd", "cinv")
+               else o.real_forward(xi) if kind == "rfwd" else o.real_inverse(xi, n))
+        assert O.split_norm_err(ys[i].cpu().numpy()[None], ref) <= 1e-5, (i, kind, n)
+    plan.close()

@@ -263,6 +263,49 @@ def test_concurrent_host_pointer_calls(torch, ctx):
     assert not errors, errors[:5]
 
 
+def test_concurrent_device_pointer_calls_overlap(torch, ctx):
+    """Eight threads sharing one context call the drop-in entry points on device buffers.
+    Each call runs on its calling thread's own default stream, so calls from different
+    threads overlap on the device (on the null stream they would queue behind each other):
+    eight threads finish 8 x 40 complex 2^15 transforms (one workgroup each) in well under
+    the time one thread needs for the same 320 calls.  Every result equals the batched
+    call's bits."""
+    import threading
+    import time
+    n, per, nthr = 1 << 15, 40, 8
+    x = make_input("cfwd", n, nthr, 6400)
+    xt = dev(torch, x)
+    want = run_batch(torch, ctx, "cfwd", n, x)
+    outs = [torch.full((2 * n,), float("nan"), device="cuda") for _ in range(nthr)]
+    torch.cuda.synchronize()
+
+    def worker(w, calls, res):
+        ok = True
+        for i in range(calls):
+            k = (w + i) % nthr if calls > per else w
+            ok &= ctx.complex_forward(n, xt[k], outs[k]) == 1
+        res[w] = ok
+
+    res = [None] * nthr
+    worker(0, nthr, res)                       # warm-up: every input once
+    t0 = time.perf_counter()
+    worker(0, nthr * per, res)
+    serial = time.perf_counter() - t0
+    assert res[0]
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker, args=(w, per, res)) for w in range(nthr)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    concurrent = time.perf_counter() - t0
+    assert all(res), res
+    for w in range(nthr):
+        assert np.array_equal(outs[w].cpu().numpy().view(np.uint32), want[w].view(np.uint32))
+    print(f"320 calls: one thread {serial * 1e3:.2f} ms, eight threads {concurrent * 1e3:.2f} ms")
+    assert concurrent < 0.7 * serial, (serial, concurrent)
+
+
 def test_device_arena_context(torch, C):
     """Tables in a caller's device arena: same bits as a library-allocated context; the
     arena is left to the caller at shutdown and can hold a new context; bad arenas refused."""

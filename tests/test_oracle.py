@@ -268,3 +268,62 @@ def test_planted_mistakes_are_caught(monkeypatch):
     Yc = O._to_c128(y)
     swapped = O.interleave(Yc[::-1]).ravel()
     assert O.split_norm_err(rest, O.fp64_real_inverse(swapped, n)) > 1e-2  # transposed k / N-k
+
+
+def _radix2_fft(x, plant=None):
+    """Iterative radix-2 decimation-in-frequency FFT in complex64 (forward), used only to
+    show that the closed-form tone check of tests/test_gpu_large.py rejects a transform
+    with a twiddle mistake.  plant: None, or (stage, kind) with kind "sign" (conjugated
+    twiddles in that stage) or "index" (twiddle index off by one in that stage)."""
+    n = x.size
+    a = x.astype(np.complex64).copy()
+    half, stage = n // 2, 0
+    while half >= 1:
+        j = np.arange(half)
+        idx = j * (n // (2 * half))
+        if plant == (stage, "index"):
+            idx = (idx + 1) % n
+        w = np.exp(-2j * np.pi * idx / n).astype(np.complex64)
+        if plant == (stage, "sign"):
+            w = np.conj(w)
+        a = a.reshape(-1, 2, half)
+        top, bot = a[:, 0, :].copy(), a[:, 1, :].copy()
+        a[:, 0, :] = top + bot
+        a[:, 1, :] = (top - bot) * w
+        a = a.reshape(n)
+        half //= 2
+        stage += 1
+    # bit reversal
+    bits = int(np.log2(n))
+    rev = np.array([int(f"{i:0{bits}b}"[::-1], 2) for i in range(n)])
+    return a[rev]
+
+
+def _tone_check(y, n, f, amp):
+    """The closed-form gate of tests/test_gpu_large.py (spectrum_gate) for a complex
+    forward transform of a sum of tones."""
+    d = y.astype(np.complex128).copy()
+    expect = n * amp
+    ref_norm = np.sqrt(np.sum(np.abs(expect) ** 2))
+    tone_err = np.abs(d[f] - expect)
+    d[f] = 0
+    return (np.all(tone_err <= 1e-6 * np.abs(expect))
+            and np.max(np.abs(d)) <= 1e-6 * ref_norm
+            and np.sqrt(np.sum(np.abs(d) ** 2) + np.sum(tone_err ** 2)) <= 1e-6 * ref_norm)
+
+
+def test_tone_check_rejects_planted_twiddle_mistakes():
+    """The large-size GPU tests check transforms of a sum of tones against its closed-form
+    spectrum on every bin; a correct FP32 FFT passes, and one wrong twiddle sign or index
+    in any single stage fails."""
+    n = 1 << 12
+    rng = np.random.default_rng(11)
+    f = rng.choice(np.arange(1, n), size=4, replace=False)
+    amp = rng.uniform(0.5, 1.0, 4) * np.exp(1j * rng.uniform(0, 2 * np.pi, 4))
+    t = np.arange(n)
+    x = sum(a * np.exp(2j * np.pi * ((fi * t) % n) / n) for fi, a in zip(f, amp))
+    assert np.allclose(_radix2_fft(x), np.fft.fft(x.astype(np.complex64)), atol=1e-2)
+    assert _tone_check(_radix2_fft(x), n, f, amp)
+    for stage in (0, 5, 10):
+        for kind in ("sign", "index"):
+            assert not _tone_check(_radix2_fft(x, (stage, kind)), n, f, amp), (stage, kind)
