@@ -3,7 +3,7 @@
 //
 // Argument validation follows the reference's C API (src/ckfft/ckfft.cpp:14-114) rule
 // for rule and returns the same 0 / 1 / NULL values; the context's size query and
-// caller-buffer mode follow src/ckfft/context.cpp:45-91.  Differences, all deliberate:
+// caller-buffer mode follow src/ckfft/context.cpp:26-67.  Differences, all deliberate:
 //  * n <= 0 is rejected (the reference's unsigned power-of-two test lets INT_MIN in);
 //  * a context pointer is also checked for a live magic value;
 //  * 0 is also returned if the GPU reports an error for the call;
@@ -281,7 +281,7 @@ CkFftContext* init_context(int nMax, CkFftDirection direction, void* buf, size_t
 
 extern "C" {
 
-// src/ckfft/ckfft.cpp:14-34 and src/ckfft/context.cpp:45-135
+// src/ckfft/ckfft.cpp:14-34 and src/ckfft/context.cpp:24-114
 CkFftContext* CkFftInit(int nMax, CkFftDirection direction, void* buf, size_t* bufSize)
 {
     return init_context(nMax, direction, buf, bufSize, nullptr, 0);
@@ -336,7 +336,7 @@ int CkFftComplexInverse(CkFftContext* context, int n, const CkFftComplex* input,
     return run_plain(context, kComplexInverse, n, input, output);
 }
 
-// src/ckfft/ckfft.cpp:116-119 and src/ckfft/context.cpp:137-143
+// src/ckfft/ckfft.cpp:116-119 and src/ckfft/context.cpp:116-122
 void CkFftShutdown(CkFftContext* context)
 {
     if (!context || context->magic != kContextMagic) return;

@@ -4,7 +4,7 @@
 //
 // Replaces the reference's recursive radix-4 decimation-in-time transform
 // (src/ckfft/fft_default.cpp:12-266) and its real pack/unpack passes
-// (src/ckfft/fft_real_default.cpp:121-222) for every size that fits one workgroup.
+// (src/ckfft/fft_real_default.cpp:13-114) for every size that fits one workgroup.
 //
 // Design (gfx950, wave64):
 //  * Each transform of N complex points is owned by T = N/P threads; every thread
@@ -25,9 +25,9 @@
 //    complex, then the pack pass out[k] = A - i W^k D, out[N-k] = conj(A + i W^k D)
 //    with A = Z[k] + conj Z[N-k], D = Z[k] - conj Z[N-k], W = exp(-2 pi i/2N), fused
 //    after the last stage.  This is the reference's post-pass
-//    (src/ckfft/fft_real_default.cpp:129-170) written for one (k, N-k) pair per step.
+//    (src/ckfft/fft_real_default.cpp:30-62) written for one (k, N-k) pair per step.
 //  * Real inverse: the unpack pre-pass t[k] = (Y[k] + conj Y[N-k]) +
-//    i conj(W^k) (Y[k] - conj Y[N-k]) (src/ckfft/fft_real_default.cpp:180-219) is
+//    i conj(W^k) (Y[k] - conj Y[N-k]) (src/ckfft/fft_real_default.cpp:79-111) is
 //    fused into the first stage's global loads, then the N-point inverse transform
 //    writes the 2N reals as N complex values.
 #pragma once

@@ -11,7 +11,7 @@ hipError_t launch_lds(Kind kind, int logc, int count, const void* in, void* out,
 
 // ---- twiddle pyramid -------------------------------------------------------------
 //
-// Replaces the reference's table loop (src/ckfft/context.cpp:111-126), which forms
+// Replaces the reference's table loop (src/ckfft/context.cpp:90-105), which forms
 // the angle in FP32 and calls cosf/sinf.  Here every entry is evaluated in FP64 with
 // an exact argument reduction and rounded once to FP32, so each stored value is the
 // correctly rounded exp(-2 pi i k / L) (up to FP64's own sub-ulp error):

@@ -8,8 +8,8 @@ Three oracles live here:
 
 * ``Restatement`` -- ctypes binding of ``libckfft_oracle.so`` (``ckfft_oracle.c``), a
   bit-exact FP32 restatement of the reference's scalar path
-  (src/ckfft/fft_default.cpp:12-266, src/ckfft/fft_real_default.cpp:121-222,
-  src/ckfft/context.cpp:111-126).
+  (src/ckfft/fft_default.cpp:12-266, src/ckfft/fft_real_default.cpp:13-114,
+  src/ckfft/context.cpp:90-105).
 * ``Reference`` -- ctypes binding of ``_ref/libckfft_ref.so``: the genuine reference built
   from its own sources by ``oracle/Makefile`` (target ``ref``).  Present only where the
   reference checkout was available at build time.
@@ -244,12 +244,12 @@ def fp64_complex(x, inverse: bool) -> np.ndarray:
 
 
 def fp64_real_forward(x) -> np.ndarray:
-    """2 * DFT(x)[0..n/2] (src/ckfft/fft_real_default.cpp:121-171 yields the factor 2)."""
+    """2 * DFT(x)[0..n/2] (src/ckfft/fft_real_default.cpp:13-63 yields the factor 2)."""
     return 2.0 * np.fft.rfft(np.asarray(x, np.float64), axis=-1)
 
 
 def fp64_real_inverse(y, n: int) -> np.ndarray:
-    """The exact linear map of src/ckfft/fft_real_default.cpp:173-222 in float64.
+    """The exact linear map of src/ckfft/fft_real_default.cpp:65-114 in float64.
 
     With h = n/2 and Y the n/2+1 inputs, the pre-pass builds for every k in [0, h]
         t[k] = (Y[k] + conj Y[h-k]) + i e^{+2 pi i k/n} (Y[k] - conj Y[h-k])

@@ -6,7 +6,7 @@
  *
  * Build: gcc -O2 -ffp-contract=off (no FMA contraction, no -ffast-math), so every
  * complex multiply is (a.re*b.re - a.im*b.im, a.im*b.re + a.re*b.im) rounded after
- * each product and each sum, as in src/ckfft/math_util.h:94-98.
+ * each product and each sum, as in src/ckfft/math_util.h:29-33.
  */
 #define _GNU_SOURCE
 #include "ckfft_oracle.h"
@@ -16,7 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* ---- FP32 complex helpers: src/ckfft/math_util.h:82-98 ---------------------- */
+/* ---- FP32 complex helpers: src/ckfft/math_util.h:17-33 ---------------------- */
 
 static inline ocf_cplx cx(float re, float im) { ocf_cplx z; z.re = re; z.im = im; return z; }
 static inline ocf_cplx cx_add(ocf_cplx a, ocf_cplx b) { return cx(a.re + b.re, a.im + b.im); }
@@ -27,11 +27,11 @@ static inline ocf_cplx cx_mul(ocf_cplx a, ocf_cplx b)
     return cx(a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im);
 }
 
-/* src/ckfft/math_util.h:77-80: the test is on the unsigned value.  The oracle also
+/* src/ckfft/math_util.h:12-15: the test is on the unsigned value.  The oracle also
  * rejects n <= 0, which the reference would let through for n == INT_MIN. */
 static int pow2_positive(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-/* ---- twiddles: src/ckfft/context.cpp:111-126 -------------------------------- */
+/* ---- twiddles: src/ckfft/context.cpp:90-105 -------------------------------- */
 
 void ocf_twiddles(int max_n, ocf_cplx* fwd, ocf_cplx* inv)
 {
@@ -142,7 +142,7 @@ static void complex_core(const ocf_ctx* c, int n, const ocf_cplx* in, ocf_cplx* 
     }
 }
 
-/* src/ckfft/fft_real.cpp:13-60 and src/ckfft/fft_real_default.cpp:121-171 */
+/* src/ckfft/fft_real.cpp:13-60 and src/ckfft/fft_real_default.cpp:13-63 */
 static void real_forward_core(const ocf_ctx* c, int n, const float* x, ocf_cplx* y)
 {
     if (n == 1) {
@@ -181,7 +181,7 @@ static void real_forward_core(const ocf_ctx* c, int n, const float* x, ocf_cplx*
     y[n / 4] = cx(mid.re * 2.0f, -mid.im * 2.0f);
 }
 
-/* src/ckfft/fft_real.cpp:62-105 and src/ckfft/fft_real_default.cpp:173-222 */
+/* src/ckfft/fft_real.cpp:62-105 and src/ckfft/fft_real_default.cpp:65-114 */
 static void real_inverse_core(const ocf_ctx* c, int n, const ocf_cplx* y, float* x, ocf_cplx* tmp)
 {
     if (n == 1) {

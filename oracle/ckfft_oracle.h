@@ -12,10 +12,10 @@
  *
  * Functions and the reference code each one restates (paths relative to the
  * reference checkout):
- *   ocf_create          src/ckfft/context.cpp:45-135   (table layout, twiddles 111-126)
+ *   ocf_create          src/ckfft/context.cpp:24-114   (table layout, twiddles 90-105)
  *   ocf_complex         src/ckfft/fft.cpp:13-46, src/ckfft/fft_default.cpp:12-266
- *   ocf_real_forward    src/ckfft/fft_real.cpp:13-60, src/ckfft/fft_real_default.cpp:121-171
- *   ocf_real_inverse    src/ckfft/fft_real.cpp:62-105, src/ckfft/fft_real_default.cpp:173-222
+ *   ocf_real_forward    src/ckfft/fft_real.cpp:13-60, src/ckfft/fft_real_default.cpp:13-63
+ *   ocf_real_inverse    src/ckfft/fft_real.cpp:62-105, src/ckfft/fft_real_default.cpp:65-114
  *   ocf_check_*         src/ckfft/ckfft.cpp:14-114 (argument validation, 0/1 returns)
  *
  * Pinned by (tests/test_oracle.py): bit-exact agreement with golden vectors produced

@@ -118,7 +118,7 @@ def main():
 
     # Twiddle tables: the reference keeps them inside its context; read them back
     # through the context layout (tables follow an 8-byte-aligned header,
-    # src/ckfft/context.cpp:45-109) via the user-buffer mode.
+    # src/ckfft/context.cpp:24-114) via the user-buffer mode.
     lib = R.lib
     for k in range(0, 25):
         m = 1 << k

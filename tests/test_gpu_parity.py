@@ -405,7 +405,7 @@ def test_argument_validation_on_device(torch, C):
 
 
 def test_caller_buffer_context(torch, C):
-    """Query mode, then a context placed in caller memory (src/ckfft/context.cpp:68-91)."""
+    """Query mode, then a context placed in caller memory (src/ckfft/context.cpp:47-67)."""
     size = ctypes.c_size_t(0)
     assert not C.CkFftInit(1024, 3, None, ctypes.byref(size))
     buf = ctypes.create_string_buffer(size.value)

@@ -2,7 +2,7 @@
 
 No compute calls here (no GPU in this environment).  CkFftInit without a GPU must
 fail cleanly with NULL rather than crash, exactly like an allocation failure in the
-reference (src/ckfft/context.cpp:82-87).
+reference (src/ckfft/context.cpp:61-66).
 """
 import ctypes
 import glob
