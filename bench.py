@@ -295,13 +295,33 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
     in_off, out_off, in_len, out_len = m["in_off"], m["out_off"], m["in_len"], m["out_len"]
     xb, yb = x.data_ptr(), y.data_ptr()
     flops = float(np.sum(m["flops"]))
+    stream = torch.cuda.current_stream(dev)
+
+    def eager_ms(c, reps=5):
+        p = C.BatchPlan(c, descs)
+        for _ in range(3):
+            p.execute(stream)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            p.execute(stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        p.close()
+        return a.elapsed_time(b) / reps
+
+    # the default layouts (no tuning), timed as the tuners time plans: eagerly, no graph
+    with C.Context(1 << 16, C.kCkFftDirection_Both) as c0:
+        default_ms = eager_ms(c0)
+    # layouts chosen in this mix (CkFftAutotuneMix from the defaults; choices made on each
+    # class alone lost in the mix on this GPU)
     ctx = C.Context(1 << 16, C.kCkFftDirection_Both)
     t0 = time.perf_counter()
-    tuned = ctx.autotune(None) == 1
-    tune_s = time.perf_counter() - t0
+    mix_ok, mix_before, mix_after = ctx.autotune_mix(descs)
+    mix_s = time.perf_counter() - t0
+    tuned, tune_s = mix_ok == 1, mix_s
     plan = C.BatchPlan(ctx, descs, graph=True)
     info = plan.info()
-    stream = torch.cuda.current_stream(dev)
     for _ in range(max(1, warmup)):
         assert plan.execute(stream) == 1
     torch.cuda.synchronize()
@@ -346,6 +366,10 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
         "pass_transforms": info["pass_transforms"],
         "work_items": info["work_items"], "workgroups": info["workgroups"],
         "extra_launches": info["extra_launches"], "autotuned": tuned, "autotune_s": round(tune_s, 2),
+        "autotune_mix": {"ok": mix_ok == 1, "s": round(mix_s, 2),
+                         "ms_per_batch_default_layouts": round(default_ms, 4),
+                         "ms_per_batch_after_mix_tuning": round(mix_after * 1e-3, 4),
+                         "timing": "plans executed eagerly (no graph): 5 executions for the default, 10 in the tuner"},
         "parity_worst_vs_ref": float(f"{worst:.3e}"),
         "plain": {"value": round(pf / plain_s / 1e9, 2), "unit": "GFLOP/s",
                   "sample": f"first {plain_sample} transforms, one synchronous call each"},

@@ -26,7 +26,7 @@ __all__ = [
     "CkFftServerWait", "CkFftServerDrain", "CkFftServerGetStats", "CkFftServerResetStats",
     "CkFftServerDestroy", "CkFftServerLastError", "Server", "CkFftSetPrecision", "CkFftGetPrecision",
     "kCkFftPrecision_Fp32", "kCkFftPrecision_SplitBf16x3", "CkFftDeviceTableBytes", "CkFftInitArena",
-    "CkFftTransformBatchNoAlloc", "NOALLOC_SCRATCH_BYTES", "EXPORTED_SYMBOLS",
+    "CkFftTransformBatchNoAlloc", "NOALLOC_SCRATCH_BYTES", "CkFftAutotuneMix", "EXPORTED_SYMBOLS",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libckfft_amd.so")
@@ -129,6 +129,8 @@ _SIGS = {
     "CkFftInitArena": (_vp, [_i, _i, _vp, ctypes.POINTER(ctypes.c_size_t), _vp, ctypes.c_size_t]),
     "CkFftGetPrecision": (_i, [_vp]),
     "CkFftTransformBatchNoAlloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "CkFftAutotuneMix": (_i, [_vp, ctypes.POINTER(CkFftBatchDesc), _i, ctypes.c_char_p,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
@@ -232,6 +234,17 @@ class Context:
 
     def autotune(self, cache_path=None) -> int:
         return CkFftAutotune(self.handle, cache_path.encode() if cache_path else None)
+
+    def autotune_mix(self, transforms, cache_path=None):
+        """CkFftAutotuneMix over (kind, n, input, output) transforms: returns (ok, us before,
+        us after) for one execution of their plan."""
+        arr = (CkFftBatchDesc * max(1, len(transforms)))()
+        for i, (kind, n, x, y) in enumerate(transforms):
+            arr[i] = CkFftBatchDesc(kind, n, _ptr(x), _ptr(y))
+        before, after = ctypes.c_double(), ctypes.c_double()
+        ok = CkFftAutotuneMix(self.handle, arr, len(transforms), cache_path.encode() if cache_path else None,
+                              ctypes.byref(before), ctypes.byref(after))
+        return ok, before.value, after.value
 
     def set_precision(self, precision: int) -> int:
         """kCkFftPrecision_Fp32 (default) or kCkFftPrecision_SplitBf16x3 (complex

@@ -125,9 +125,13 @@ struct _CkFftContext {
     void* d_tables;          // the one device block holding the three tables above
     int own_device;          // 1 if the library allocated d_tables (else a caller's arena)
     int log_max;             // largest log2(n) the tables serve
-    // Tuned variant (points per thread 16 or 32) per [kind][log2 complex size], used by
-    // batch plans; every variant has the same arithmetic.
-    unsigned char tune[4][16];
+    // Batch-plan layout per [kind][log2 complex size] (CkFftAutotune, CkFftAutotuneMix);
+    // every layout has the same arithmetic.  Sizes 2^1..2^15: 0 = the default, else
+    // 1 + var + 2 * wide, var = 32 instead of 16 points per thread (2^5..2^13), wide = the
+    // one-workgroup-per-CU persistent kernel instead of the two-per-CU one.  [kind][16]
+    // (complex 2^16): 0 = the table-driven passes over all such transforms, 2 = an ordinary
+    // two-pass launch each.
+    unsigned char tune[4][17];
     // kCkFftPrecision_*: FP32 (default) or split-BF16x3 matrix-core radix-16 stages for
     // the complex sizes that mode covers.
     int precision;

@@ -64,6 +64,32 @@ __host__ __device__ constexpr bool narrow_class(int logc, int var)
 
 __host__ __device__ constexpr int lds_cls(int kind, int logc, int var) { return (kind * 16 + logc) * 2 + var; }
 
+// The layout of a class in a batch plan: its points-per-thread variant and its queue.
+struct ClassLayout {
+    int var;
+    bool wide;
+};
+
+// Default: 16 points per thread; the narrow kernel for the classes of 2^11..2^14 points
+// that fit it, the wide kernel for the others.
+static ClassLayout default_layout(int logc, int var)
+{
+    return ClassLayout{var, !(narrow_class(logc, var) && logc >= 11)};
+}
+
+// tune: the context's code for the class (see _CkFftContext::tune); var_override >= 0
+// forces a variant with the default queue rule.  Codes a class cannot take fall back to
+// what it can (var 1 only for 2^5..2^13; the narrow kernel only for narrow_class()).
+static ClassLayout class_layout(int logc, int tune, int var_override)
+{
+    if (var_override >= 0) return default_layout(logc, var_override);
+    if (tune <= 0) return default_layout(logc, 0);
+    ClassLayout l{(tune - 1) & 1, ((tune - 1) >> 1) != 0};
+    if (persist_logp(logc, 0) == persist_logp(logc, 1)) l.var = 0;
+    if (!l.wide && !narrow_class(logc, l.var)) l.wide = true;
+    return l;
+}
+
 template <bool NARROW, int KIND, int LOGC, int VAR>
 __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
                                                   const float2* __restrict__ tw, float* lds)
@@ -281,7 +307,7 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
         // as ordinary launches of that mode's kernels
         if (ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n)) ok = false;
         const bool is_twopass = ok && !real && log2n == kPassLogN;
-        if (is_twopass && two_pass == TwoPass::kTable) {
+        if (is_twopass && two_pass == TwoPass::kTable && ctx->tune[d.kind][kPassLogN] != 2) {
             (d.kind == kComplexInverse ? out.pass_inv : out.pass_fwd)
                 .push_back(BatchTransform{(const float2*)d.input, (float2*)d.output, nullptr});
         } else if (ok && !is_twopass) {
@@ -304,13 +330,14 @@ bool batch_build(const CkFftContext* ctx, const CkFftBatchDesc* descs, int count
     }
     for (size_t i = 0; i < pers.size();) {
         const Entry& e = pers[i];
-        const int var = var_override >= 0 ? var_override : ctx->tune[e.kind][e.logc];
+        const ClassLayout lay = class_layout(e.logc, ctx->tune[e.kind][e.logc], var_override);
+        const int var = lay.var;
         const int T = 1 << (e.logc - persist_logp(e.logc, var));
         const int tpw = kPersistWG / T;
         size_t j = i;
         while (j < pers.size() && pers[j].kind == e.kind && pers[j].logc == e.logc && (int)(j - i) < tpw) ++j;
         const int cnt = (int)(j - i);
-        (narrow && narrow_class(e.logc, var) && e.logc >= 11 ? work_n : work)
+        (narrow && !lay.wide ? work_n : work)
             .push_back({cnt * transform_bytes(e.kind, e.log2n), BatchItem{lds_cls(e.kind, e.logc, var), (int)i, cnt, 0}});
         i = j;
     }
@@ -582,14 +609,83 @@ void CkFftBatchPlanDestroy(CkFftBatchPlan* plan)
 
 const char* CkFftBatchLastError(void) { return g_batch_error; }
 
+}  // extern "C"
+
 // ---- autotuner ------------------------------------------------------------------------
 
 static const char* kKindTag[4] = {"cfwd", "cinv", "rfwd", "rinv"};
+
+namespace {
+
+// The layout codes a class can take (see _CkFftContext::tune), the default first.
+std::vector<int> class_codes(int logc)
+{
+    std::vector<int> c{0};
+    if (logc == kPassLogN) {                 // complex 2^16: table passes or ordinary launches
+        c.push_back(2);
+        return c;
+    }
+    const bool two_vars = persist_logp(logc, 0) != persist_logp(logc, 1);
+    for (int var = 0; var < (two_vars ? 2 : 1); ++var)
+        for (int wide = 0; wide < 2; ++wide) {
+            if (!wide && !narrow_class(logc, var)) continue;
+            const ClassLayout d = default_layout(logc, 0);
+            if (var == d.var && (wide != 0) == d.wide) continue;   // the default, already listed
+            c.push_back(1 + var + 2 * wide);
+        }
+    return c;
+}
+
+// Microseconds per execution of a plan over descs (3 untimed, then `reps` timed).
+float time_plan(CkFftContext* ctx, const CkFftBatchDesc* d, int count, int reps, hipEvent_t e0, hipEvent_t e1,
+                bool* ok)
+{
+    CkFftBatchPlan* p = plan_create(ctx, d, count, 0, -1);
+    if (!p) {
+        *ok = false;
+        return 0.f;
+    }
+    for (int w = 0; w < 3; ++w) CkFftBatchPlanExecute(p, nullptr);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) CkFftBatchPlanExecute(p, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    CkFftBatchPlanDestroy(p);
+    return ms * 1000.f / reps;
+}
+
+bool write_tuning(const CkFftContext* ctx, const char* path, const std::string& times)
+{
+    std::string json = "{\n \"arch\": \"gfx950\",\n \"format\": 2,\n \"choice\": {";
+    bool first = true;
+    for (int kind = 0; kind < 4; ++kind)
+        for (int logc = 1; logc <= kPassLogN; ++logc) {
+            if (!ctx->tune[kind][logc]) continue;
+            char buf[64];
+            std::snprintf(buf, sizeof(buf), "%s\n  \"%s_%d\": %d", first ? "" : ",", kKindTag[kind],
+                          kind >= kRealForward ? logc + 1 : logc, ctx->tune[kind][logc]);
+            json += buf;
+            first = false;
+        }
+    json += "},\n \"us\": {" + times + "}\n}\n";
+    FILE* f = std::fopen(path, "w");
+    if (!f) return false;
+    std::fputs(json.c_str(), f);
+    std::fclose(f);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
 
 int CkFftAutotune(CkFftContext* context, const char* cachePath)
 {
     if (!context || context->magic != kContextMagic) return 0;
     const size_t kBytesPerClass = 32u << 20;
+    const int kMaxTransforms = 20000;
     void* in = nullptr;
     void* out = nullptr;
     if (!batch_error(hipMalloc(&in, kBytesPerClass), "hipMalloc(tune in)") ||
@@ -602,19 +698,21 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath)
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    std::string json = "{\n \"arch\": \"gfx950\",\n \"choice\": {";
-    std::string times = "},\n \"us_per_call\": {";
-    bool first_entry = true;
-    int ok = 1;
+    std::string times;
+    bool ok = true;
     for (int kind = 0; kind < 4 && ok; ++kind) {
         const int need_dir = (kind == kComplexForward || kind == kRealForward) ? kCkFftDirection_Forward
                                                                                : kCkFftDirection_Inverse;
         if (!(context->direction & need_dir)) continue;
-        for (int logc = 5; logc <= 13; ++logc) {
+        for (int logc = 5; logc <= kPassLogN && ok; ++logc) {
             const int log2n = kind >= kRealForward ? logc + 1 : logc;
             if (log2n > context->log_max) continue;
+            if (kind >= kRealForward && logc == kPassLogN) continue;   // real 2^17: not in batch plans' passes
+            const std::vector<int> codes = class_codes(logc);
+            if (codes.size() < 2) continue;
             const long long tb = transform_bytes(kind, log2n);
-            const int count = (int)std::max(1ll, (long long)(kBytesPerClass / 2) / tb);
+            const int count = (int)std::min((long long)kMaxTransforms,
+                                            std::max(1ll, (long long)(kBytesPerClass / 2) / tb));
             const long long n = 1ll << log2n;
             const long long in_stride = kind == kRealForward ? 4 * n : (kind == kRealInverse ? 8 * (n / 2 + 1) : 8 * n);
             const long long out_stride = kind == kRealForward ? 8 * (n / 2 + 1) : (kind == kRealInverse ? 4 * n : 8 * n);
@@ -622,39 +720,23 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath)
             for (int i = 0; i < count; ++i)
                 d[i] = CkFftBatchDesc{kind, (int)n, (const char*)in + i * in_stride, (char*)out + i * out_stride};
             float best = 1e30f;
-            int best_var = 0;
-            float t_var[2] = {0, 0};
-            for (int var = 0; var < 2; ++var) {
-                CkFftBatchPlan* p = plan_create(context, d.data(), count, 0, var);
-                if (!p) {
-                    ok = 0;
-                    break;
-                }
-                for (int w = 0; w < 3; ++w) CkFftBatchPlanExecute(p, nullptr);
-                (void)hipEventRecord(e0, nullptr);
-                const int reps = 10;
-                for (int r = 0; r < reps; ++r) CkFftBatchPlanExecute(p, nullptr);
-                (void)hipEventRecord(e1, nullptr);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                CkFftBatchPlanDestroy(p);
-                t_var[var] = ms * 1000.f / reps;
-                if (t_var[var] < best) {
-                    best = t_var[var];
-                    best_var = var;
+            int best_code = 0;
+            std::string t = "[";
+            for (size_t c = 0; c < codes.size() && ok; ++c) {
+                context->tune[kind][logc] = (unsigned char)codes[c];
+                const float us = time_plan(context, d.data(), count, 10, e0, e1, &ok);
+                char buf[48];
+                std::snprintf(buf, sizeof(buf), "%s[%d, %.2f]", c ? ", " : "", codes[c], us);
+                t += buf;
+                if (us < best) {
+                    best = us;
+                    best_code = codes[c];
                 }
             }
-            if (!ok) break;
-            context->tune[kind][logc] = (unsigned char)best_var;
-            char buf[160];
-            std::snprintf(buf, sizeof(buf), "%s\n  \"%s_%d\": %d", first_entry ? "" : ",", kKindTag[kind], log2n,
-                          best_var);
-            json += buf;
-            std::snprintf(buf, sizeof(buf), "%s\n  \"%s_%d\": [%.2f, %.2f]", first_entry ? "" : ",", kKindTag[kind],
-                          log2n, t_var[0], t_var[1]);
-            times += buf;
-            first_entry = false;
+            context->tune[kind][logc] = (unsigned char)best_code;
+            char key[48];
+            std::snprintf(key, sizeof(key), "%s\n  \"%s_%d\": ", times.empty() ? "" : ",", kKindTag[kind], log2n);
+            times += key + t + "]";
         }
     }
     (void)hipEventDestroy(e0);
@@ -662,14 +744,85 @@ int CkFftAutotune(CkFftContext* context, const char* cachePath)
     (void)hipFree(in);
     (void)hipFree(out);
     if (!ok) return 0;
-    json += times + "}\n}\n";
-    if (cachePath) {
-        FILE* f = std::fopen(cachePath, "w");
-        if (!f) return 0;
-        std::fputs(json.c_str(), f);
-        std::fclose(f);
+    return cachePath ? (write_tuning(context, cachePath, times) ? 1 : 0) : 1;
+}
+
+int CkFftAutotuneMix(CkFftContext* context, const CkFftBatchDesc* descs, int count, const char* cachePath,
+                     double* usBefore, double* usAfter)
+{
+    if (!context || context->magic != kContextMagic || count <= 0 || !descs) return 0;
+    // the classes present, largest share of the bytes first
+    std::vector<std::pair<long long, int>> cls;   // (bytes, kind * 32 + logc)
+    for (int i = 0; i < count; ++i) {
+        if (!batch_check_desc(context, descs[i], i, true)) return 0;
+        const int log2n = ilog2(descs[i].n);
+        const int logc = descs[i].kind >= kRealForward ? log2n - 1 : log2n;
+        if (logc < 1 || logc > kPassLogN) continue;
+        const int key = descs[i].kind * 32 + logc;
+        auto it = std::find_if(cls.begin(), cls.end(), [&](const std::pair<long long, int>& c) { return c.second == key; });
+        const long long b = transform_bytes(descs[i].kind, log2n);
+        if (it == cls.end())
+            cls.push_back({b, key});
+        else
+            it->first += b;
     }
-    return 1;
+    std::sort(cls.begin(), cls.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    bool ok = true;
+    const int reps = 10;
+    float cur = time_plan(context, descs, count, reps, e0, e1, &ok);
+    const float before = cur;
+    // start from the faster of the context's choices and the defaults (choices made on
+    // classes alone can lose in a mix, where the two persistent launches share the GPU)
+    unsigned char saved[4][kPassLogN + 1];
+    std::memcpy(saved, context->tune, sizeof(saved));
+    std::memset(context->tune, 0, sizeof(saved));
+    const float dflt = ok ? time_plan(context, descs, count, reps, e0, e1, &ok) : 0.f;
+    if (dflt < cur)
+        cur = dflt;
+    else
+        std::memcpy(context->tune, saved, sizeof(saved));
+    std::string times;
+    // coordinate descent, two sweeps: each class tries its other layouts in the whole mix
+    // and keeps one that is more than 0.5% faster than the current layout
+    for (int sweep = 0; sweep < 2 && ok; ++sweep)
+        for (size_t k = 0; k < cls.size() && ok; ++k) {
+            const int kind = cls[k].second / 32, logc = cls[k].second % 32;
+            const std::vector<int> codes = class_codes(logc);
+            if (codes.size() < 2 || (kind >= kRealForward && logc == kPassLogN)) continue;
+            const int start = context->tune[kind][logc];
+            int best_code = start;
+            float best = cur;
+            std::string t = "[";
+            for (size_t c = 0; c < codes.size() && ok; ++c) {
+                if (codes[c] == start) continue;
+                context->tune[kind][logc] = (unsigned char)codes[c];
+                const float us = time_plan(context, descs, count, reps, e0, e1, &ok);
+                char buf[48];
+                std::snprintf(buf, sizeof(buf), "%s[%d, %.1f]", t.size() > 1 ? ", " : "", codes[c], us);
+                t += buf;
+                if (us < 0.995f * best) {
+                    best = us;
+                    best_code = codes[c];
+                }
+            }
+            context->tune[kind][logc] = (unsigned char)best_code;
+            cur = best;
+            char key[64];
+            std::snprintf(key, sizeof(key), "%s\n  \"%s_%d (sweep %d)\": ", times.empty() ? "" : ",", kKindTag[kind],
+                          kind >= kRealForward ? logc + 1 : logc, sweep);
+            times += key + t + "]";
+        }
+    // the final layout, timed again
+    const float after = ok ? time_plan(context, descs, count, reps, e0, e1, &ok) : 0.f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) return 0;
+    if (usBefore) *usBefore = before;
+    if (usAfter) *usAfter = after;
+    return cachePath ? (write_tuning(context, cachePath, times) ? 1 : 0) : 1;
 }
 
 int CkFftLoadTuning(CkFftContext* context, const char* cachePath)
@@ -682,23 +835,27 @@ int CkFftLoadTuning(CkFftContext* context, const char* cachePath)
     size_t got;
     while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
     std::fclose(f);
+    const bool v2 = text.find("\"format\": 2") != std::string::npos;
     const size_t c0 = text.find("\"choice\"");
     if (c0 == std::string::npos) return 0;
     const size_t c1 = text.find('}', c0);
     const std::string sec = text.substr(c0, c1 - c0);
     int loaded = 0;
     for (int kind = 0; kind < 4; ++kind)
-        for (int logc = 5; logc <= 13; ++logc) {
+        for (int logc = 1; logc <= kPassLogN; ++logc) {
             const int log2n = kind >= kRealForward ? logc + 1 : logc;
             char key[32];
             std::snprintf(key, sizeof(key), "\"%s_%d\":", kKindTag[kind], log2n);
             const size_t k = sec.find(key);
             if (k == std::string::npos) continue;
-            const int v = std::atoi(sec.c_str() + k + std::strlen(key));
-            if (v == 0 || v == 1) {
-                context->tune[kind][logc] = (unsigned char)v;
-                ++loaded;
-            }
+            int v = std::atoi(sec.c_str() + k + std::strlen(key));
+            // the first format held the variant only (0: 16 points per thread, 1: 32, which
+            // always ran in the wide kernel)
+            if (!v2) v = v == 1 ? 4 : 0;
+            const std::vector<int> codes = class_codes(logc);
+            if (std::find(codes.begin(), codes.end(), v) == codes.end()) continue;
+            context->tune[kind][logc] = (unsigned char)v;
+            ++loaded;
         }
     return loaded > 0 ? 1 : 0;
 }

@@ -185,7 +185,6 @@ __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t,
     constexpr int N = PL::N;
     constexpr int R = 1 << PL::rlog(0);
     constexpr int B = PL::P / R;
-    constexpr int M = N / R;
     static_assert(B == 2 && 2 * R == 64, "one mirror pair of butterflies per thread, 2R lanes in a wave");
     const float2* lvl = tw + 2 * N;
     const float2 wl = lvl[t];                           // W^t; W^(t + rM) = W^t exp(-2 pi i r/2R)

@@ -107,11 +107,26 @@ int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info);
 void CkFftBatchPlanDestroy(CkFftBatchPlan* plan);
 const char* CkFftBatchLastError(void);
 
-/* Time both thread layouts of every class (kind, 32 <= n <= 8192 complex points) on
- * this GPU, keep the faster in the context (used by later plans) and, if cachePath is
- * not NULL, write them as JSON.  Layouts differ only in speed, never in results. */
+/* Choose each class's layout in batch plans on this GPU.  A class is (kind, n); its layouts
+ * are the points per thread (16 or 32, for 32 <= n <= 8192 complex points -- which sets the
+ * waves per transform and the transforms per workgroup), the persistent kernel that runs it
+ * (one workgroup per CU with a 136 KiB LDS tile, or two per CU with half that, for the
+ * classes that fit; 16384-point transforms then exchange one LDS plane at a time) and, for
+ * complex n = 65536, table-driven passes over all such transforms or an ordinary two-pass
+ * launch each.  The radix plan is fixed per size: it sets the arithmetic, and layouts differ
+ * only in speed, never in results.  CkFftAutotune times every layout of every class (complex
+ * 32 .. 65536 points, real 64 .. 65536 samples) in a plan of that class alone and keeps the
+ * fastest in the context (used by later plans); if cachePath is not NULL it writes the
+ * choices and times as JSON. */
 int CkFftAutotune(CkFftContext* context, const char* cachePath);
-/* Load choices written by CkFftAutotune.  1 if any entry was applied. */
+/* Choose the layouts inside a given mix: starting from the faster of the context's choices
+ * and the defaults, for each class of descs (largest share of the bytes first, two sweeps)
+ * time the whole plan with each of the class's other layouts and keep one that is more than
+ * 0.5% faster.  usBefore / usAfter (may be NULL) receive the plan's microseconds per
+ * execution with the context's choices on entry and with the chosen ones.  1/0. */
+int CkFftAutotuneMix(CkFftContext* context, const CkFftBatchDesc* descs, int count, const char* cachePath,
+                     double* usBefore, double* usAfter);
+/* Load choices written by CkFftAutotune or CkFftAutotuneMix.  1 if any entry was applied. */
 int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
 
 /* ---- serving loop ------------------------------------------------------------------

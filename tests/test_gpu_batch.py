@@ -139,7 +139,10 @@ def test_autotune_changes_speed_not_results(torch, C, tmp_path):
         path = str(tmp_path / "tune.json")
         assert c.autotune(path) == 1
         data = json.load(open(path))
-        assert data["arch"] == "gfx950" and len(data["choice"]) == 4 * 9
+        # every class with more than one layout was timed: complex 2^5..2^14 and 2^16, real
+        # 2^6..2^15 samples; each choice is a valid code
+        assert data["arch"] == "gfx950" and data["format"] == 2 and len(data["us"]) == 2 * 11 + 2 * 10
+        assert all(1 <= v <= 4 for v in data["choice"].values())
         mix = random_mix(80, 14, kmin=5, kmax=14)
         ins, outs, descs = build(torch, C, c, mix, 4000)
         plan = C.BatchPlan(c, descs)
@@ -294,3 +297,58 @@ def test_config_uneven_full_20000(torch, C, ctx):
                else o.real_forward(xi) if kind == "rfwd" else o.real_inverse(xi, n))
         assert O.split_norm_err(ys[i].cpu().numpy()[None], ref) <= 1e-5, (i, kind, n)
     plan.close()
+
+
+def test_every_layout_code_equals_single_calls(torch, C, tmp_path):
+    """A tuning file (second format) with a non-default layout for every class: the small
+    classes in the narrow kernel, 2^11..2^14 in the wide one (2^14 then with both LDS planes
+    at once), and complex 2^16 as ordinary two-pass launches instead of the table passes.
+    Every output still equals the single call bit for bit."""
+    keys = []
+    for k in KINDS:
+        for c in range(1 if k[0] == "c" else 2, 15):
+            code = 1 if c <= 10 else 3               # var 0: narrow for the small, wide for the rest
+            keys.append(f'"{k}_{(c + 1) if k[0] == "r" else c}": {code}')
+    keys += ['"cfwd_16": 2', '"cinv_16": 2']
+    path = tmp_path / "every_code.json"
+    path.write_text('{\n "arch": "gfx950",\n "format": 2,\n "choice": {' + ", ".join(keys) + "}\n}\n")
+    mix = [(k, 1 << e) for k in KINDS for e in range(1 if k[0] == "c" else 3, 17 if k[0] == "c" else 17)]
+    with C.Context(1 << 16) as c:
+        assert c.load_tuning(str(path)) == 1
+        ins, outs, descs = build(torch, C, c, mix + mix, 7000)
+        plan = C.BatchPlan(c, descs)
+        info = plan.info()
+        assert info["pass_transforms"] == 0                  # complex 2^16 as ordinary launches
+        assert info["extra_launches"] == sum(1 for k, n in mix + mix if (k[0] == "c" and n in (1, 1 << 16))
+                                             or (k[0] == "r" and n <= 4))
+        assert plan.execute() == 1
+        torch.cuda.synchronize()
+        for i, (kind, n) in enumerate(mix + mix):
+            want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
+            assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
+        plan.close()
+
+
+def test_autotune_mix_keeps_results(torch, C, tmp_path):
+    """CkFftAutotuneMix over an uneven mix times the whole plan with each class's other
+    layouts; the tuned plan's outputs equal the single calls bit for bit."""
+    rng = np.random.default_rng(0x7E)
+    mix = []
+    for _ in range(600):
+        k = int(rng.integers(5, 17))
+        real, fwd = rng.random() < 0.5, rng.random() < 0.5
+        mix.append((("rfwd" if fwd else "rinv") if real else ("cfwd" if fwd else "cinv"), 1 << k))
+    with C.Context(1 << 16) as c:
+        ins, outs, descs = build(torch, C, c, mix, 7500)
+        path = str(tmp_path / "mix.json")
+        ok, before, after = c.autotune_mix(descs, path)
+        assert ok == 1 and before > 0 and after > 0
+        data = json.load(open(path))
+        assert data["format"] == 2 and len(data["us"]) > 0
+        plan = C.BatchPlan(c, descs, graph=True)
+        assert plan.execute() == 1
+        torch.cuda.synchronize()
+        for i, (kind, n) in enumerate(mix):
+            want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
+            assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
+        plan.close()
