@@ -429,6 +429,10 @@ def serving_leg(torch, C, dev, rate: float = 200e3, seconds: float = 2.0, budget
                     f"50% complex/real, 50% fwd/inv), HBM budget {budget / 2**30:g} GiB, {slots} slots",
         "requests": count, "completed": st["completed"], "failed": st["failed"],
         "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
+        "max_request": {"client_us": round(st["max_client_us"], 1), "wait_us": round(st["max_wait_us"], 1),
+                        "device_us": round(st["max_device_us"], 1),
+                        "meaning": "the slowest request: scheduled arrival to submission (the replaying client), "
+                                   "submission to its batch's launch, launch to completion seen"},
         "mean_us": round(st["mean_us"], 1),
         "throughput_req_s": round(st["requests_per_s"], 0), "offered_req_s": rate,
         "achieved_GBs": round(st["GB_per_s"], 2), "offered_GBs": round(rate * offered_bytes / 1e9, 2),

@@ -85,7 +85,8 @@ class CkFftServerStats(ctypes.Structure):
                 ("requests_per_s", ctypes.c_double), ("GB_per_s", ctypes.c_double),
                 ("peak_inflight_bytes", ctypes.c_longlong), ("budget_bytes", ctypes.c_longlong),
                 ("resident_bytes", ctypes.c_longlong), ("batches", ctypes.c_longlong),
-                ("mean_batch", ctypes.c_double)]
+                ("mean_batch", ctypes.c_double), ("max_client_us", ctypes.c_double),
+                ("max_wait_us", ctypes.c_double), ("max_device_us", ctypes.c_double)]
 
 
 _vp = ctypes.c_void_p

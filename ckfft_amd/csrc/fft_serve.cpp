@@ -10,9 +10,12 @@
 //    uploaded asynchronously from the slot's pinned buffers -- so batches formed while
 //    others run overlap on the device;
 //  * polls the slots' completion events, finishes their requests and frees budget.
-// Complex 2^16 requests (above one workgroup's LDS) run as ordinary multi-pass launches
-// in the batch's stream, their intermediate charged to the budget with the request.
-// Twiddles, descriptor buffers and streams stay resident for the server's life.  A request's result does not depend on the batch it ran in: every kernel that
+// Requests above one workgroup's LDS (complex n >= 2^16, real n >= 2^17) run as ordinary
+// multi-pass launches in the batch's stream, one after another, with their intermediate in
+// the slot's resident buffer (up to 64 MiB; a larger one is allocated on the stream and
+// charged to the budget with the request), so the serving path allocates nothing.
+// Twiddles, descriptor buffers, intermediates and streams stay resident for the server's
+// life.  A request's result does not depend on the batch it ran in: every kernel that
 // computes a transform rounds the same way.
 #include <hip/hip_runtime.h>
 
@@ -50,6 +53,8 @@ struct Request {
     CkFftBatchDesc d{};
     long long bytes = 0;     // algorithmic bytes, charged against the budget
     uint64_t t_arrive = 0;   // arrival (submission or scheduled arrival) time
+    uint64_t t_submit = 0;   // when the submission reached the server
+    uint64_t t_launch = 0;   // launch of its batch
     long long ticket = -1;   // the submission this entry holds
     int state = kDone;
 };
@@ -66,12 +71,20 @@ struct Slot {
     char* h_desc = nullptr;              // pinned staging of the batch's queue and transform table
     char* d_desc = nullptr;              // their device copy
     int* d_ctl = nullptr;
+    // intermediate of the multi-pass transforms of the slot's batches: they run one after
+    // another on the slot's stream, so one buffer serves them all and none allocates
+    float2* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
     std::vector<long long> reqs;
     long long bytes = 0;
     bool busy = false;
 };
 
 constexpr int kMaxBatch = 2048;          // transforms per batch (slot buffer capacity)
+// A slot's resident intermediate covers the multi-pass sizes up to this many bytes (8n for a
+// complex transform, 4n for a real one: complex 2^23 or real 2^24); larger requests allocate
+// theirs on the slot's stream and charge it to the budget.
+constexpr size_t kSlotScratchCap = (size_t)64 << 20;
 constexpr uint64_t kSpinNs = 2000000;    // idle dispatcher spins this long before sleeping
 // queue entries first, the transform table after them at a 256-byte boundary, so one
 // copy uploads both
@@ -101,6 +114,10 @@ struct _CkFftServer {
     size_t lat_next = 0;
     long long completed = 0, failed = 0, batches = 0, batched = 0, done_bytes = 0;
     uint64_t t_first = 0, t_last = 0;
+    // the slowest request since the reset: its latency, scheduled arrival to submission
+    // (the client's delay), submission to launch, launch to completion seen
+    double max_us = 0, max_client_us = 0, max_wait_us = 0, max_device_us = 0;
+    size_t slot_scratch = 0;             // bytes of each slot's resident intermediate
     // tickets of failed requests, kept past the reuse of their ring entries so that a query
     // of an old ticket still reports its failure (failures are rare: a launch error)
     std::unordered_set<long long> failed_tickets;
@@ -126,6 +143,12 @@ void finish_slot(CkFftServer* S, Slot& sl, bool ok)
             Request& r = S->ring[id % kRing];
             if (ok) {
                 const double us = (double)(t - r.t_arrive) * 1e-3;
+                if (us > S->max_us) {
+                    S->max_us = us;
+                    S->max_client_us = r.t_submit > r.t_arrive ? (double)(r.t_submit - r.t_arrive) * 1e-3 : 0.0;
+                    S->max_wait_us = (double)(r.t_launch - std::max(r.t_arrive, r.t_submit)) * 1e-3;
+                    S->max_device_us = (double)(t - r.t_launch) * 1e-3;
+                }
                 if (S->lat_us.size() < kLatencyWindow)
                     S->lat_us.push_back(us);
                 else
@@ -177,15 +200,24 @@ bool launch_batch(CkFftServer* S, Slot& sl, std::vector<long long>& batch, const
                                         std::min(S->grid, (int)b.items.size()), tb, sl.stream, false),
                   "persistent launch");
     }
-    for (const auto& e : b.extras)
-        if (ok) ok = serr(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, sl.stream), "extra launch");
+    for (const auto& e : b.extras) {
+        const bool fits = transform_scratch_bytes(e.kind, e.log2n, 1, tb) <= sl.scratch_bytes;
+        if (ok)
+            ok = serr(launch_transform(e.kind, e.log2n, 1, e.in, e.out, tb, sl.stream, fits ? sl.d_scratch : nullptr,
+                                       fits ? sl.scratch_bytes : 0),
+                      "extra launch");
+    }
     if (ok) ok = serr(hipEventRecord(sl.done, sl.stream), "record");
     sl.reqs.swap(batch);
     sl.bytes = bytes;
     sl.busy = true;
     {
         std::lock_guard<std::mutex> lock(S->m);
-        for (long long id : sl.reqs) S->ring[id % kRing].state = kLaunched;
+        const uint64_t t = now_ns();
+        for (long long id : sl.reqs) {
+            S->ring[id % kRing].state = kLaunched;
+            S->ring[id % kRing].t_launch = t;
+        }
         ++S->batches;
         S->batched += (long long)sl.reqs.size();
     }
@@ -285,6 +317,7 @@ void destroy_slots(CkFftServer* S)
         if (sl.h_desc) (void)hipHostFree(sl.h_desc);
         if (sl.d_desc) (void)hipFree(sl.d_desc);
         if (sl.d_ctl) (void)hipFree(sl.d_ctl);
+        if (sl.d_scratch) (void)hipFree(sl.d_scratch);
     }
     S->slots.clear();
 }
@@ -300,6 +333,10 @@ CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxI
     if (!S) return nullptr;
     S->ctx = context;
     S->slots.resize(maxInFlight);
+    // the largest multi-pass intermediate the context's sizes need (complex nMax >= 2^16:
+    // 8 nMax bytes; real nMax >= 2^17: 4 nMax), up to the cap
+    const size_t nmax = (size_t)context->max_count;
+    S->slot_scratch = nmax >= ((size_t)1 << (kLogMaxLds + 1)) ? std::min(8 * nmax, kSlotScratchCap) : 0;
     for (Slot& sl : S->slots) {
         if (!serr(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "stream") ||
             !serr(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "event") ||
@@ -312,6 +349,15 @@ CkFftServer* CkFftServerCreate(CkFftContext* context, size_t hbmBudget, int maxI
             return nullptr;
         }
         S->resident += (long long)(kDescBytes + sizeof(int) * kCtlWords);
+        if (S->slot_scratch) {
+            if (!serr(hipMalloc((void**)&sl.d_scratch, S->slot_scratch), "slot intermediate")) {
+                destroy_slots(S);
+                delete S;
+                return nullptr;
+            }
+            sl.scratch_bytes = S->slot_scratch;
+            S->resident += (long long)S->slot_scratch;
+        }
     }
     S->budget = (long long)hbmBudget - S->resident;
     S->grid = persistent_grid(context, false);
@@ -334,12 +380,13 @@ static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_a
     }
     int log2n = 0;
     while ((1 << log2n) < d.n) ++log2n;
-    // algorithmic bytes, plus the intermediate a multi-pass launch allocates for itself
+    // algorithmic bytes, plus the intermediate of a multi-pass launch when the slot's resident
+    // one is too small for it (one half-size complex signal, 8 * n/2 bytes, for a real kind)
     const bool real = d.kind >= kRealForward;
     const bool multipass = real ? log2n > kLogMaxLds + 1 : log2n > kLogMaxLds;
     const bool mfma = S->ctx->precision == kCkFftPrecision_SplitBf16x3 && mfma_supported((Kind)d.kind, log2n);
-    // (one half-size complex signal, 8 * n/2 bytes, for a real kind)
-    const long long bytes = transform_bytes(d.kind, log2n) + (multipass && !mfma ? (real ? 4ll : 8ll) << log2n : 0);
+    const long long inter = multipass && !mfma ? (real ? 4ll : 8ll) << log2n : 0;
+    const long long bytes = transform_bytes(d.kind, log2n) + (inter > (long long)S->slot_scratch ? inter : 0);
     if (bytes > S->budget) {
         std::snprintf(g_serve_error, sizeof(g_serve_error), "request larger than the admission budget");
         return -1;
@@ -354,6 +401,7 @@ static long long submit_at(CkFftServer* S, const CkFftBatchDesc& d, uint64_t t_a
         r.d = d;
         r.bytes = bytes;
         r.t_arrive = t_arrive;
+        r.t_submit = now_ns();
         r.ticket = id;
         r.state = kQueued;
         S->queued.push_back(id);
@@ -439,6 +487,9 @@ int CkFftServerGetStats(CkFftServer* server, CkFftServerStats* out)
     out->peak_inflight_bytes = server->peak_inflight;
     out->budget_bytes = server->budget;
     out->resident_bytes = server->resident;
+    out->max_client_us = server->max_client_us;
+    out->max_wait_us = server->max_wait_us;
+    out->max_device_us = server->max_device_us;
     out->batches = server->batches;
     out->mean_batch = server->batches ? (double)server->batched / server->batches : 0.0;
     return 1;
@@ -453,6 +504,7 @@ void CkFftServerResetStats(CkFftServer* server)
     server->completed = server->failed = server->batches = server->batched = server->done_bytes = 0;
     server->peak_inflight = server->inflight_bytes;
     server->t_first = server->t_last = 0;
+    server->max_us = server->max_client_us = server->max_wait_us = server->max_device_us = 0;
 }
 
 void CkFftServerDestroy(CkFftServer* server)

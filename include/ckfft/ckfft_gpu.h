@@ -139,10 +139,11 @@ int CkFftLoadTuning(CkFftContext* context, const char* cachePath);
  * arriving while others run join the next batch and their work overlaps on the device.
  * Admission is first-come-first-served against an HBM budget: a request is admitted
  * while the bytes of admitted, unfinished requests (one read of the input, one write of
- * the output, plus the intermediate of the multi-pass sizes: 8n bytes complex, 4n real)
- * plus the server's own resident buffers stay within hbmBudget; others wait
- * (back-pressure, not failure).
- * Twiddles, descriptor buffers and streams stay resident.
+ * the output) plus the server's own resident buffers stay within hbmBudget; others wait
+ * (back-pressure, not failure).  Each slot holds a resident intermediate for the
+ * multi-pass sizes (8n bytes complex, 4n real, up to 64 MiB: the context's nMax at most);
+ * a request that needs more has its intermediate charged with it.
+ * Twiddles, descriptor buffers, intermediates and streams stay resident.
  * A request's result is bit-identical to the single-signal call on the same input,
  * whatever it was batched with.  Buffers are device memory owned by the caller and
  * must stay valid until the request completes.  The reference has no counterpart. */
@@ -161,6 +162,9 @@ typedef struct {
     long long resident_bytes;     /* the server's own device buffers */
     long long batches;            /* batches launched */
     double mean_batch;            /* requests per batch */
+    double max_client_us;         /* the slowest request (max_us): arrival to its submission */
+    double max_wait_us;           /* its submission to its batch's launch */
+    double max_device_us;         /* its batch's launch to completion seen */
 } CkFftServerStats;
 
 /* NULL on invalid arguments, a budget below the server's resident buffers, or
