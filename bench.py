@@ -297,9 +297,9 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
     flops = float(np.sum(m["flops"]))
     stream = torch.cuda.current_stream(dev)
 
-    def eager_ms(c, reps=5):
+    def eager_ms(c, reps=10):
         p = C.BatchPlan(c, descs)
-        for _ in range(3):
+        for _ in range(10):
             p.execute(stream)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
@@ -369,7 +369,7 @@ def uneven_leg(torch, C, O, dev, steps: int, warmup: int, plain_sample: int = 20
         "autotune_mix": {"ok": mix_ok == 1, "s": round(mix_s, 2),
                          "ms_per_batch_default_layouts": round(default_ms, 4),
                          "ms_per_batch_after_mix_tuning": round(mix_after * 1e-3, 4),
-                         "timing": "plans executed eagerly (no graph): 5 executions for the default, 10 in the tuner"},
+                         "timing": "plans executed eagerly (no graph), 10 executions each after warm-up"},
         "parity_worst_vs_ref": float(f"{worst:.3e}"),
         "plain": {"value": round(pf / plain_s / 1e9, 2), "unit": "GFLOP/s",
                   "sample": f"first {plain_sample} transforms, one synchronous call each"},
@@ -743,6 +743,14 @@ def main():
         torch.cuda.synchronize()
         return a.elapsed_time(b) / reps
 
+    # the same K steps on one buffer set (cache-resident between steps), for comparison
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(0)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    ms_per_step_cached = t_start.elapsed_time(t_end) / args.steps
+
     fwd_launch = lambda k: ctx.real_forward_batch(n, B, xs[k], ys[k], stream)
     inv_launch = lambda k: ctx.real_inverse_batch(n, B, ys[k], zs[k], stream)
     fwd_ms = back_to_back(fwd_launch)
@@ -787,7 +795,9 @@ def main():
                          f"cycling through {nsets} buffer sets ({nsets * set_bytes / 2**30:.2f} GiB)",
         "step_GBs": round(2 * bytes_launch / (ms_per_step * 1e-3) / 1e9, 1),
         "frac_per_launch": {"real_forward": frac(fwd_ms), "real_inverse": frac(inv_ms)},
-        "in_cache": {"launch_ms": {"real_forward": round(fwd_ms_cached, 5), "real_inverse": round(inv_ms_cached, 5)},
+        "in_cache": {"ms_per_step": round(ms_per_step_cached, 5),
+                     "GFLOP_s": round(flops_step / (ms_per_step_cached * 1e-3) / 1e9, 2),
+                     "launch_ms": {"real_forward": round(fwd_ms_cached, 5), "real_inverse": round(inv_ms_cached, 5)},
                      "frac_per_launch": {"real_forward": frac(fwd_ms_cached), "real_inverse": frac(inv_ms_cached)},
                      "working_set_bytes": set_bytes,
                      "timing": "the same launches on one buffer set (Infinity Cache resident between launches)"},
