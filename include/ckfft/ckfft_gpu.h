@@ -94,14 +94,18 @@ typedef struct {
 typedef struct _CkFftBatchPlan CkFftBatchPlan;
 
 /* Flag: capture the plan's launches (the persistent kernels and the table-driven passes)
- * in a hipGraph once and replay it. */
+ * in a hipGraph once and replay it.  With or without it, an execution runs the table-driven
+ * passes, the two-per-CU persistent launch and the one-per-CU persistent launch as three
+ * branches side by side, on streams the plan owns, forked from and joined back into the
+ * caller's stream. */
 #define kCkFftBatch_Graph 1
 
 /* Validate the descriptors (same rules as the single-signal calls, plus device
  * pointers) and build the queue.  NULL on invalid arguments or allocation failure
  * (CkFftBatchLastError tells which). */
 CkFftBatchPlan* CkFftBatchPlanCreate(CkFftContext* context, const CkFftBatchDesc* descs, int count, int flags);
-/* Enqueue one execution on `stream` (a hipStream_t; NULL = null stream).  1/0. */
+/* Enqueue one execution on `stream` (a hipStream_t; NULL = null stream): work after it on
+ * `stream` sees the outputs.  Executions of one plan must not overlap.  1/0. */
 int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream);
 int CkFftBatchPlanGetInfo(const CkFftBatchPlan* plan, CkFftBatchInfo* info);
 void CkFftBatchPlanDestroy(CkFftBatchPlan* plan);
@@ -121,9 +125,11 @@ const char* CkFftBatchLastError(void);
 int CkFftAutotune(CkFftContext* context, const char* cachePath);
 /* Choose the layouts inside a given mix: starting from the faster of the context's choices
  * and the defaults, for each class of descs (largest share of the bytes first, two sweeps)
- * time the whole plan with each of the class's other layouts and keep one that is more than
- * 0.5% faster.  usBefore / usAfter (may be NULL) receive the plan's microseconds per
- * execution with the context's choices on entry and with the chosen ones.  1/0. */
+ * time the whole plan (graph replays) with each of the class's other layouts against the
+ * current layouts, in alternating rounds, and keep the best that is more than 1% faster.
+ * The result is timed the same way against the layouts on entry and kept only if faster.
+ * usBefore / usAfter (may be NULL) receive the plan's microseconds per execution with the
+ * layouts on entry and with those kept (never more).  1/0; on 0 the entry layouts stay. */
 int CkFftAutotuneMix(CkFftContext* context, const CkFftBatchDesc* descs, int count, const char* cachePath,
                      double* usBefore, double* usAfter);
 /* Load choices written by CkFftAutotune or CkFftAutotuneMix.  1 if any entry was applied. */

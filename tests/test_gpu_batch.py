@@ -331,7 +331,8 @@ def test_every_layout_code_equals_single_calls(torch, C, tmp_path):
 
 def test_autotune_mix_keeps_results(torch, C, tmp_path):
     """CkFftAutotuneMix over an uneven mix times the whole plan with each class's other
-    layouts; the tuned plan's outputs equal the single calls bit for bit."""
+    layouts; it never leaves layouts slower than those on entry, and the tuned plan's
+    outputs equal the single calls bit for bit."""
     rng = np.random.default_rng(0x7E)
     mix = []
     for _ in range(600):
@@ -342,7 +343,7 @@ def test_autotune_mix_keeps_results(torch, C, tmp_path):
         ins, outs, descs = build(torch, C, c, mix, 7500)
         path = str(tmp_path / "mix.json")
         ok, before, after = c.autotune_mix(descs, path)
-        assert ok == 1 and before > 0 and after > 0
+        assert ok == 1 and before > 0 and 0 < after <= before
         data = json.load(open(path))
         assert data["format"] == 2 and len(data["us"]) > 0
         plan = C.BatchPlan(c, descs, graph=True)
