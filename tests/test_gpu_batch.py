@@ -353,3 +353,33 @@ def test_autotune_mix_keeps_results(torch, C, tmp_path):
             want = plain(torch, c, kind, n, ins[i]).cpu().numpy()
             assert np.array_equal(outs[i].cpu().numpy().view(np.uint32), want.view(np.uint32)), (i, kind, n)
         plan.close()
+
+
+def test_plan_branches_join_the_callers_stream(torch, C, ctx):
+    """A plan runs its table passes, its two-per-CU persistent launch and its one-per-CU
+    persistent launch side by side, on streams of its own forked from the caller's stream.
+    Work queued on the caller's stream after an execution must see every output of every
+    branch: the outputs are cleared and snapshotted on that stream around each execution,
+    eagerly and as graph replays, and every snapshot must equal the single calls bit for
+    bit."""
+    mix = ([("cfwd", 1 << 16)] * 3 + [("cinv", 1 << 16)] * 2 + [("rfwd", 1 << 16)] * 6 +
+           [("rinv", 1 << 16)] * 4 + [("cinv", 4096)] * 24 + [("rfwd", 2048)] * 24 + [("cfwd", 256)] * 8)
+    ins, outs, descs = build(torch, C, ctx, mix, 9100)
+    want = [plain(torch, ctx, kind, n, ins[i]).cpu().numpy() for i, (kind, n) in enumerate(mix)]
+    s = torch.cuda.Stream()
+    for graph in (False, True):
+        plan = C.BatchPlan(ctx, descs, graph=graph)
+        info = plan.info()
+        assert info["pass_transforms"] == 5 and info["persistent_transforms"] == len(mix) - 5
+        for rep in range(3):
+            with torch.cuda.stream(s):
+                for y in outs:
+                    y.fill_(float("nan"))
+            assert plan.execute(s) == 1
+            with torch.cuda.stream(s):
+                snap = [y.clone() for y in outs]
+            torch.cuda.synchronize()
+            for i, (kind, n) in enumerate(mix):
+                got = snap[i].cpu().numpy()
+                assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (graph, rep, i, kind, n)
+        plan.close()
