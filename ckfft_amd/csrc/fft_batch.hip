@@ -91,6 +91,10 @@ static ClassLayout class_layout(int logc, int tune, int var_override)
     return l;
 }
 
+// The persistent kernels read their transforms' inputs and write their outputs with the
+// non-temporal hint, as the stand-alone kernels do.
+constexpr bool kPersistNt = true;
+
 template <bool NARROW, int KIND, int LOGC, int VAR>
 __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTransform* __restrict__ trs,
                                                   const float2* __restrict__ tw, float* lds)
@@ -111,7 +115,7 @@ __device__ __forceinline__ void run_lds_item(const BatchItem& w, const BatchTran
     const int slot = SH::TPW == 1 ? 0 : tid / SH::T;
     const bool active = slot < w.count;
     const BatchTransform d = trs[w.first + (active ? slot : 0)];
-    lds_transform<LOGC, LP, kPersistWG, KIND, SPLIT>(d.in, d.out, active, tw, lds, tid);
+    lds_transform<LOGC, LP, kPersistWG, KIND, SPLIT, kPersistNt>(d.in, d.out, active, tw, lds, tid);
 }
 
 // Classes whose two variants coincide (n <= 16 or n >= 2^14) are instantiated once; the
@@ -388,10 +392,10 @@ __global__ __launch_bounds__(PassShape<kPassLogR>::WG) void fft_pass_table_kerne
     constexpr int logns = FIRST ? 0 : kPassLogN - kPassLogR;
     static_assert(XTW == 0 || XTW == 2, "one twiddle source per kernel");
     if (g < n_fwd)   // uniform over the workgroup
-        pass_tile<kPassLogR, SH::LOGCB, false, FIRST, XTW>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
+        pass_tile<kPassLogR, SH::LOGCB, false, FIRST, XTW, !FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
                                                            lds, threadIdx.x);
     else
-        pass_tile<kPassLogR, SH::LOGCB, true, FIRST, XTW>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
+        pass_tile<kPassLogR, SH::LOGCB, true, FIRST, XTW, !FIRST>(src, dst, q, kPassLogN, logns, pyr, fine, xtw, log_xtw,
                                                           lds, threadIdx.x);
 }
 

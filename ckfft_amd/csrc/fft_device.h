@@ -10,6 +10,49 @@
 
 namespace ckfft_gpu {
 
+// ---- a transform's own input and output ------------------------------------------------
+// A transform reads each input value once and writes each output value once.  With NT
+// these accesses carry the non-temporal hint, so that streaming a batch through the
+// caches does not displace what is reused (twiddle tables, the intermediate of the next
+// pass) and the written results leave without competing with later reads.  Measured from
+// HBM (buffer sets beyond the 256 MB Infinity Cache): complex 2^24 170 -> 136 us (reads of
+// each pass), real 65536 x 256 forward + inverse step 88 -> 77 us.
+typedef float NtF2 __attribute__((ext_vector_type(2)));
+typedef float NtF4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ float2 ld_in(const float2* p)
+{
+    if constexpr (NT) {
+        const NtF2 u = __builtin_nontemporal_load(reinterpret_cast<const NtF2*>(p));
+        return make_float2(u.x, u.y);
+    } else {
+        return *p;
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_out(float2* p, float2 v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(NtF2{v.x, v.y}, reinterpret_cast<NtF2*>(p));
+    else
+        *p = v;
+}
+
+template <bool NT>
+__device__ __forceinline__ void st_out4(float4* p, float4 v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(NtF4{v.x, v.y, v.z, v.w}, reinterpret_cast<NtF4*>(p));
+    else
+        *p = v;
+}
+
+// Cache-policy operand of the raw buffer stores: 2 is the non-temporal bit on gfx950.
+template <bool NT>
+constexpr int kStoreAux = NT ? 2 : 0;
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 // a*b with one explicit fused multiply-add per component.  All HIP code is compiled with

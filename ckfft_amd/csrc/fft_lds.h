@@ -104,7 +104,7 @@ __device__ __forceinline__ float2 edge_twiddle(const float2* __restrict__ lvl, i
 // First-stage loads.  They carry no per-element predicate: a workgroup slot past the
 // last signal reads a valid signal instead (see the kernel) and never stores, so every
 // load issues back to back.
-template <class PL>
+template <class PL, bool NT>
 __device__ __forceinline__ void load_first_complex(float2* v, int t, const float2* __restrict__ src)
 {
     constexpr int R = 1 << PL::rlog(0);
@@ -112,7 +112,7 @@ __device__ __forceinline__ void load_first_complex(float2* v, int t, const float
 #pragma unroll
     for (int b = 0; b < B; ++b)
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b * R + r] = src[in_index<PL, 0>(t + b * PL::T, r)];
+        for (int r = 0; r < R; ++r) v[b * R + r] = ld_in<NT>(&src[in_index<PL, 0>(t + b * PL::T, r)]);
 }
 
 // Real-inverse unpack of one point: t[k] = (Y[k] + conj Y[N-k]) + i e^{+2 pi i k/2N} (Y[k] - conj Y[N-k]),
@@ -178,7 +178,7 @@ __device__ __forceinline__ void wave_lds_sync()
 // their 2R original values in `part` (LDS outside the exchange planes), the 2R lanes of
 // its wave unpack one element each there, and thread 0 takes the results back after the
 // in-place pass, so no lane needs extra registers and no lane unpacks 2R points alone.
-template <class PL, bool INV>
+template <class PL, bool INV, bool NT>
 __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t, const float2* __restrict__ y,
                                                                const float2* __restrict__ tw, float* part)
 {
@@ -190,8 +190,8 @@ __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t,
     const float2 wl = lvl[t];                           // W^t; W^(t + rM) = W^t exp(-2 pi i r/2R)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        v[r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, 0), r)];
-        v[R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, 1), r)];
+        v[r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, 0), r)]);
+        v[R + r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, 1), r)]);
     }
     float2* pp = reinterpret_cast<float2*>(part);
     if (t < 2 * R) {                                    // thread 0's wave
@@ -232,7 +232,7 @@ __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t,
 // The loads are issued pair by pair (slots b and b+H, then thread 0's partners after the
 // first pair), ahead of the arithmetic; each pair is then unpacked and its two
 // butterflies computed.
-template <class PL, bool INV>
+template <class PL, bool INV, bool NT>
 __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, const float2* __restrict__ y,
                                                           const float2* __restrict__ tw, float* part)
 {
@@ -242,7 +242,7 @@ __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, cons
     constexpr int H = B / 2;
     constexpr int M = N / R;
     if constexpr (R > 8) {
-        first_stage_unpack_mirror_wide<PL, INV>(v, t, y, tw, part);
+        first_stage_unpack_mirror_wide<PL, INV, NT>(v, t, y, tw, part);
         return;
     }
     const float2* lvl = tw + 2 * N;
@@ -254,8 +254,8 @@ __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, cons
     for (int b = 0; b < H; ++b) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            v[b * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b), r)];
-            v[(b + H) * R + r] = y[in_index<PL, 0>(bfly<PL, 0, true>(t, b + H), r)];
+            v[b * R + r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, b), r)]);
+            v[(b + H) * R + r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, b + H), r)]);
         }
         if (b == 0 && t == 0) {
             // butterflies 0 and M/2: point r*M pairs with N - r*M, point M/2 + (R-1-r)*M
@@ -293,7 +293,7 @@ __device__ __forceinline__ void first_stage_unpack_mirror(float2* v, int t, cons
 
 // ---- last-stage stores -----------------------------------------------------------
 
-template <class PL>
+template <class PL, bool NT>
 __device__ __forceinline__ void store_last(const float2* v, int t, float2* __restrict__ dst, bool active)
 {
     constexpr int S = PL::NSTAGES - 1;
@@ -304,7 +304,7 @@ __device__ __forceinline__ void store_last(const float2* v, int t, float2* __res
     for (int b = 0; b < B; ++b)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
+            st_out<NT>(&dst[out_index<PL, S>(t + b * PL::T, r)], v[b * R + r]);
 }
 
 // One real-forward output: out[k] = A - i W^k D, A = Z[k] + conj Z[N-k], D = Z[k] - conj Z[N-k].
@@ -318,12 +318,12 @@ __device__ __forceinline__ float2 pack_one(float2 zk, float2 zm, float2 w)
 
 // Both outputs of the pair (k, N-k), each formed exactly as pack_one forms it;
 // zk = Z[k], zm = Z[N-k mod N].  R is the edge radix of edge_twiddle.
-template <int N, int R>
+template <int N, int R, bool NT>
 __device__ __forceinline__ void pack_pair(int k, float2 zk, float2 zm, const float2* __restrict__ lvl,
                                           float2* __restrict__ dst)
 {
-    dst[k] = pack_one(zk, zm, edge_twiddle<N, R>(lvl, k));
-    if (k != N - k) dst[N - k] = pack_one(zm, zk, edge_twiddle<N, R>(lvl, N - k));
+    st_out<NT>(&dst[k], pack_one(zk, zm, edge_twiddle<N, R>(lvl, k)));
+    if (k != N - k) st_out<NT>(&dst[N - k], pack_one(zm, zk, edge_twiddle<N, R>(lvl, N - k)));
 }
 
 // The signal base of a workgroup that owns one transform, in scalar registers.
@@ -388,7 +388,7 @@ __device__ __forceinline__ void pack_pair_cse(float2 zk, float2 zm, float2 w, fl
     om = make_float2(a.x - c.y, -(a.y + c.x));
 }
 
-template <class PL, bool INV, bool UNIFORM>
+template <class PL, bool INV, bool UNIFORM, bool NT>
 __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const float2* __restrict__ tw,
                                                        StageTwiddles<PL, PL::NSTAGES - 1, INV, true>& tws,
                                                        float2 (&wl)[PL::P / (1 << PL::rlog(PL::NSTAGES - 1)) / 2],
@@ -452,12 +452,12 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
             if (t != 0 && active) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    dst[t + r * M] = v[r];
-                    dst[(M - t) + (R - 1 - r) * M] = v[R + R - 1 - r];
+                    st_out<NT>(&dst[t + r * M], v[r]);
+                    st_out<NT>(&dst[(M - t) + (R - 1 - r) * M], v[R + R - 1 - r]);
                 }
             }
-            if (t < 2 * R && active) dst[spk] = sp;
-            if (t == 0 && active) dst[N] = nyq;
+            if (t < 2 * R && active) st_out<NT>(&dst[spk], sp);
+            if (t == 0 && active) st_out<NT>(&dst[N], nyq);
             return;
         }
         float2* stg = reinterpret_cast<float2*>(stage);   // the exchange planes are free now
@@ -481,7 +481,7 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
                 // 64 points a wave stores are 4 whole lines
                 const unsigned mis = (unsigned)(((unsigned long long)(dst + c0)) & 127u);
                 const int a = c0 + (mis ? (int)((128u - mis) >> 3) : 0);
-                if (t < a - c0) dst[c0 + t] = stg[t];
+                if (t < a - c0) st_out<NT>(&dst[c0 + t], stg[t]);
                 const int cnt = c1 - a;
                 // pairs of points as 16-byte stores (dst + a is on a line): half the store
                 // instructions of one point per lane, forward 41.4 -> 40.7 us from HBM
@@ -491,9 +491,9 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
                     const int i = t + j * PL::T;
                     if (2 * i + 1 < cnt) {
                         const float2 u0 = stg[a - c0 + 2 * i], u1 = stg[a - c0 + 2 * i + 1];
-                        dst4[i] = make_float4(u0.x, u0.y, u1.x, u1.y);
+                        st_out4<NT>(&dst4[i], make_float4(u0.x, u0.y, u1.x, u1.y));
                     } else if (2 * i < cnt) {
-                        dst[a + 2 * i] = stg[a - c0 + 2 * i];
+                        st_out<NT>(&dst[a + 2 * i], stg[a - c0 + 2 * i]);
                     }
                 }
             }
@@ -540,16 +540,16 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
                 const int c0 = b * PL::T + r * M;
                 const int c1 = b == 0 ? (R - 1 - r) * M : (H - 1 - b) * PL::T + (R - 1 - r) * M;
                 __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, pack_one(lo, p0, w0)), rs, vo0,
-                                                      c0 * (int)sizeof(float2), 0);
+                                                      c0 * (int)sizeof(float2), kStoreAux<NT>);
                 __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, pack_one(hi, p1, w1)), rs,
-                                                      b == 0 ? voh0 : vom, c1 * (int)sizeof(float2), 0);
+                                                      b == 0 ? voh0 : vom, c1 * (int)sizeof(float2), kStoreAux<NT>);
             } else {
-                dst[k0] = pack_one(lo, p0, w0);
-                dst[k1] = pack_one(hi, p1, w1);
+                st_out<NT>(&dst[k0], pack_one(lo, p0, w0));
+                st_out<NT>(&dst[k1], pack_one(hi, p1, w1));
             }
         }
     }
-    if (active && t == 0) dst[N] = pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N));   // the Nyquist output
+    if (active && t == 0) st_out<NT>(&dst[N], pack_one(v[0], v[0], edge_twiddle<N, R>(lvl, N)));   // the Nyquist output
 }
 
 // Every output index of stage S (no mirror) equals IndexSplit's U(t) + C(b, r) and lies in
@@ -574,7 +574,7 @@ constexpr bool store_split_exact()
 // signal, each index split as IndexSplit's per-thread part (a VGPR byte offset, computed
 // once) plus its compile-time part (a scalar offset), so no store needs 64-bit address
 // arithmetic.  The resource spans the signal's N points; every index is below N.
-template <class PL, bool INV, bool UNIFORM>
+template <class PL, bool INV, bool UNIFORM, bool NT>
 __device__ __forceinline__ void last_stage_store(float2* v, int t,
                                                  StageTwiddles<PL, PL::NSTAGES - 1, INV, false>& tws,
                                                  float2* __restrict__ dst, bool active)
@@ -596,7 +596,7 @@ __device__ __forceinline__ void last_stage_store(float2* v, int t,
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(StoreBits, v[b * R + r]), rs, vo,
-                                                          IS::C(b, r) * (int)sizeof(float2), 0);
+                                                          IS::C(b, r) * (int)sizeof(float2), kStoreAux<NT>);
             }
         }
     } else {
@@ -605,7 +605,7 @@ __device__ __forceinline__ void last_stage_store(float2* v, int t,
             butterfly_compute<PL, S, INV, false>(v, tws, b);
             if (active) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) dst[out_index<PL, S>(t + b * PL::T, r)] = v[b * R + r];
+                for (int r = 0; r < R; ++r) st_out<NT>(&dst[out_index<PL, S>(t + b * PL::T, r)], v[b * R + r]);
             }
         }
     }
@@ -613,7 +613,7 @@ __device__ __forceinline__ void last_stage_store(float2* v, int t,
 
 // Real forward epilogue when the last stage cannot be mirror-closed: Z goes through
 // LDS, then thread t packs pairs k = t + q*T for k in [0, N/2].
-template <class PL, bool SPLIT>
+template <class PL, bool SPLIT, bool NT>
 __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float* im,
                                               const float2* __restrict__ tw, float2* __restrict__ dst,
                                               bool active)
@@ -665,7 +665,7 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int k = t + q * T;
-        if (k <= N / 2) pack_pair<N, RE>(k, zk[q], zm[q], lvl, dst);
+        if (k <= N / 2) pack_pair<N, RE, NT>(k, zk[q], zm[q], lvl, dst);
     }
 }
 
@@ -708,7 +708,9 @@ struct LdsShape {
 // `dst` (that signal's buffers), with `lds` holding LdsShape::LDS_WORDS floats.  A slot
 // with `active` false computes (on a valid signal) but stores nothing, so every load
 // issues unpredicated.  Used by the batch kernel below and by the persistent kernel.
-template <int LOGN, int LOGP, int WG, int KIND, bool SPLIT>
+// NT: the transform's input reads and output writes carry the non-temporal hint (see
+// ld_in / st_out).
+template <int LOGN, int LOGP, int WG, int KIND, bool SPLIT, bool NT = true>
 __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, float2* __restrict__ dst, bool active,
                                               const float2* __restrict__ tw, float* lds, int tid)
 {
@@ -733,29 +735,29 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
         if constexpr (KIND == kRealInverse)
             load_first_unpack<PL>(v, t, src, tw);
         else
-            load_first_complex<PL>(v, t, src);
+            load_first_complex<PL, NT>(v, t, src);
         stage_compute<PL, 0, INV>(v, t, tw);
         if constexpr (KIND == kRealForward) {
             if (active) {
                 const float2* lvl = tw + 2 * N;
 #pragma unroll
                 for (int k = 0; k <= N / 2; ++k)
-                    pack_pair<N, N>(k, v[k & (N - 1)], v[(N - k) & (N - 1)], lvl, dst);
+                    pack_pair<N, N, NT>(k, v[k & (N - 1)], v[(N - k) & (N - 1)], lvl, dst);
             }
         } else {
-            store_last<PL>(v, t, dst, active);
+            store_last<PL, NT>(v, t, dst, active);
         }
     } else {
         static_assert(PL::NSTAGES >= 2, "a transform over several threads has at least two stages");
         constexpr int SL = PL::NSTAGES - 1;
         // first stage, fused with the loads (and the real-inverse unpack)
         if constexpr (KIND == kRealInverse && MS == 0) {
-            first_stage_unpack_mirror<PL, INV>(v, t, src, tw, lds + SH::PART_OFFSET);
+            first_stage_unpack_mirror<PL, INV, NT>(v, t, src, tw, lds + SH::PART_OFFSET);
         } else if constexpr (KIND == kRealInverse) {
             load_first_unpack<PL>(v, t, src, tw);
             stage_compute<PL, 0, INV>(v, t, tw);
         } else {
-            load_first_complex<PL>(v, t, src);
+            load_first_complex<PL, NT>(v, t, src);
             stage_compute<PL, 0, INV>(v, t, tw);
         }
         // middle stages, then the exchange into the last one
@@ -769,22 +771,22 @@ __device__ __forceinline__ void lds_transform(const float2* __restrict__ src, fl
 #pragma unroll
             for (int b = 0; b < H; ++b) wl[b] = tw[2 * N + t + b * T];
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, true>(v, t, re, im);
-            last_stage_pack_mirror<PL, INV, SH::TPW == 1>(v, t, tw, tws, wl, dst, active, lds + SH::PART_OFFSET,
+            last_stage_pack_mirror<PL, INV, SH::TPW == 1, NT>(v, t, tw, tws, wl, dst, active, lds + SH::PART_OFFSET,
                                                           lds);
         } else if constexpr (KIND == kRealForward) {
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
             stage_compute<PL, SL, INV>(v, t, tw);
-            pack_epilogue<PL, SPLIT>(v, t, re, im, tw, dst, active);
+            pack_epilogue<PL, SPLIT, NT>(v, t, re, im, tw, dst, active);
         } else if constexpr (T < 32) {
             // several transforms per half-wave (small sizes): reading the twiddles ahead
             // measured slower here
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
             stage_compute<PL, SL, INV>(v, t, tw);
-            store_last<PL>(v, t, dst, active);
+            store_last<PL, NT>(v, t, dst, active);
         } else {
             StageTwiddles<PL, SL, INV, false> tws(t, tw);
             exchange<PL, SL - 1, SL, SPLIT, MS == SL - 1, false>(v, t, re, im);
-            last_stage_store<PL, INV, SH::TPW == 1>(v, t, tws, dst, active);
+            last_stage_store<PL, INV, SH::TPW == 1, NT>(v, t, tws, dst, active);
         }
     }
 }

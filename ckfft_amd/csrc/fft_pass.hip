@@ -26,7 +26,8 @@ namespace ckfft_gpu {
 // XTW_PASS: the host has checked that the transposed twiddles cover this pass (span
 // 2^kLogXtwCols, level <= 2^log_xtw; possible for radix <= 2^8 only); otherwise the
 // twiddles come from the pyramid and fine tables.  One code path per kernel.
-template <int LOGR, bool INV, bool FIRST, bool XTW_PASS>
+// LAST: the pass writes the transform's output (with the non-temporal hint).
+template <int LOGR, bool INV, bool FIRST, bool XTW_PASS, bool LAST>
 __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ pyr,
     const float2* __restrict__ fine, const float2* __restrict__ xtw, int log_xtw, int logn, int logns,
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_kernel(
     const long long N = 1ll << logn;
     static_assert(!XTW_PASS || LOGR + kLogXtwCols <= kLogMaxPyramid, "transposed twiddles cover radix <= 2^8");
     constexpr int XTW = XTW_PASS ? 2 : 0;
-    pass_tile<LOGR, SH::LOGCB, INV, FIRST, XTW>(in + g * N, out + g * N, q, logn, logns, pyr, fine, xtw, log_xtw,
+    pass_tile<LOGR, SH::LOGCB, INV, FIRST, XTW, LAST>(in + g * N, out + g * N, q, logn, logns, pyr, fine, xtw, log_xtw,
                                                 lds, threadIdx.x);
 }
 
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_unpack_kernel(
 #pragma unroll
     for (int b = 0; b < B0; ++b)
 #pragma unroll
-        for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = ys[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols];
+        for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = ld_in<true>(&ys[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols]);
     const float2 y_last = ys[h];   // the mirror of point 0
     // every lane's values into its column's LDS row, then each point's mirror back out
     float* row = lds + cA * SH::RSTRIDE;
@@ -152,8 +153,8 @@ __device__ __forceinline__ void pack_out(float2* __restrict__ o, long long k, lo
     const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);
     const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
     const float2 c = cmul(w, d);
-    o[k] = make_float2(a.x + c.y, a.y - c.x);
-    if (k != h - k) o[h - k] = make_float2(a.x - c.y, -(a.y + c.x));
+    st_out<true>(&o[k], make_float2(a.x + c.y, a.y - c.x));
+    if (k != h - k) st_out<true>(&o[h - k], make_float2(a.x - c.y, -(a.y + c.x)));
 }
 
 // Real forward: the last pass of the half-size transform (span ncols = h/R) writing the
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(PassShape<LOGR>::WG) void fft_pass_pack_kernel(
 #pragma unroll
     for (int b = 0; b < B0; ++b)
 #pragma unroll
-        for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = zs[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols];
+        for (int rr = 0; rr < R0; ++rr) v[b * R0 + rr] = ld_in<true>(&zs[col + (long long)in_index<PL, 0>(tA + b * T, rr) * ncols]);
     ptw.apply(v);
     float* reA = lds + cA * SH::RSTRIDE;
     float* imA = reA + SH::SPAN;
@@ -264,33 +265,38 @@ struct PassCfg {
     int cb;
 };
 
-template <int LOGR, bool INV, bool FIRST, bool XTW_PASS>
+template <int LOGR, bool INV, bool FIRST, bool XTW_PASS, bool LAST>
 constexpr PassCfg pass_cfg()
 {
     // radices above 2^8 have no transposed-twiddle variant
     constexpr bool X = XTW_PASS && LOGR + kLogXtwCols <= kLogMaxPyramid;
-    return PassCfg{fft_pass_kernel<LOGR, INV, FIRST, X>, PassShape<LOGR>::WG, PassShape<LOGR>::CB};
+    return PassCfg{fft_pass_kernel<LOGR, INV, FIRST, X, LAST>, PassShape<LOGR>::WG, PassShape<LOGR>::CB};
 }
 
-template <bool INV, bool FIRST, bool XTW_PASS, int... L>
+template <bool INV, bool FIRST, bool XTW_PASS, bool LAST, int... L>
 constexpr void fill_pass(PassCfg* tab, std::integer_sequence<int, L...>)
 {
-    ((tab[L + kLogMinPass] = pass_cfg<L + kLogMinPass, INV, FIRST, XTW_PASS>()), ...);
+    ((tab[L + kLogMinPass] = pass_cfg<L + kLogMinPass, INV, FIRST, XTW_PASS, LAST>()), ...);
 }
 
 // Pass kernels by [inverse][variant][log2 radix]; variant 0 = first pass, 1 = later pass,
-// 2 = later pass with twiddles from the transposed copy.
+// 2 = later pass with twiddles from the transposed copy, 3 and 4 = the same two as the
+// last pass (a first pass is never the last).
 struct PassTables {
-    PassCfg t[2][3][kLogMaxPass + 1];
+    PassCfg t[2][5][kLogMaxPass + 1];
     PassTables()
     {
         constexpr auto L = std::make_integer_sequence<int, kLogMaxPass - kLogMinPass + 1>{};
-        fill_pass<false, true, false>(t[0][0], L);
-        fill_pass<false, false, false>(t[0][1], L);
-        fill_pass<false, false, true>(t[0][2], L);
-        fill_pass<true, true, false>(t[1][0], L);
-        fill_pass<true, false, false>(t[1][1], L);
-        fill_pass<true, false, true>(t[1][2], L);
+        fill_pass<false, true, false, false>(t[0][0], L);
+        fill_pass<false, false, false, false>(t[0][1], L);
+        fill_pass<false, false, true, false>(t[0][2], L);
+        fill_pass<false, false, false, true>(t[0][3], L);
+        fill_pass<false, false, true, true>(t[0][4], L);
+        fill_pass<true, true, false, false>(t[1][0], L);
+        fill_pass<true, false, false, false>(t[1][1], L);
+        fill_pass<true, false, true, false>(t[1][2], L);
+        fill_pass<true, false, false, true>(t[1][3], L);
+        fill_pass<true, false, true, true>(t[1][4], L);
     }
 };
 
@@ -357,11 +363,11 @@ static bool xtw_covers(const DeviceTables& tb, int logr, int logns)
     return tb.xtw && logns == kLogXtwCols && logns + logr <= tb.log_xtw && logr + kLogXtwCols <= kLogMaxPyramid;
 }
 
-static hipError_t launch_pass(int logr, bool inv, bool first, const float2* in, float2* out, const DeviceTables& tb,
-                              int logc, int logns, long long count, hipStream_t s)
+static hipError_t launch_pass(int logr, bool inv, bool first, bool last, const float2* in, float2* out,
+                              const DeviceTables& tb, int logc, int logns, long long count, hipStream_t s)
 {
     const bool xtw = !first && xtw_covers(tb, logr, logns);
-    const PassCfg& c = pass_tables().t[inv][first ? 0 : (xtw ? 2 : 1)][logr];
+    const PassCfg& c = pass_tables().t[inv][first ? 0 : (xtw ? 2 : 1) + (last ? 2 : 0)][logr];
     const long long tiles = ((1ll << logc) >> logr) / c.cb;
     const long long total = tiles * count;
     if (total > 0x7fffffffll) return hipErrorInvalidValue;
@@ -405,7 +411,7 @@ static hipError_t run_passes(Kind edge, bool inv, int logc, long long count, con
                                    tb.xtw, tb.log_xtw, logn_real, tiles, total);
             e = hipGetLastError();
         } else {
-            e = launch_pass(logr[p], inv, p == 0, src, dst, tb, logc, logns, count, s);
+            e = launch_pass(logr[p], inv, p == 0, p == np - 1, src, dst, tb, logc, logns, count, s);
         }
         if (e != hipSuccess) return e;
         src = dst;
