@@ -216,6 +216,7 @@ struct LdsIO {
 
 // Whole signals in global memory: column b is signal g0 + b of 2^LOGN points.  A short
 // last group reads the last signal again and stores nothing for columns past `count`.
+// Reads and writes carry the non-temporal hint (ld_in / st_out): each is done once.
 template <int LOGN>
 struct SignalIO {
     static constexpr bool kLds = false;
@@ -230,7 +231,7 @@ struct SignalIO {
         const long long g = g0 + b < count ? g0 + b : count - 1;
         const float2* p = in + g * N + idx0;
 #pragma unroll
-        for (int e = 0; e < CNT; ++e) x[e] = p[e * STRIDE];
+        for (int e = 0; e < CNT; ++e) x[e] = ld_in<true>(&p[e * STRIDE]);
     }
     template <int CNT>
     __device__ __forceinline__ void store_run(int b, int idx0, const float2* v) const
@@ -238,7 +239,7 @@ struct SignalIO {
         if (g0 + b >= count) return;
         float2* p = out + (g0 + b) * N + idx0;
 #pragma unroll
-        for (int e = 0; e < CNT; ++e) p[e] = v[e];
+        for (int e = 0; e < CNT; ++e) st_out<true>(&p[e], v[e]);
     }
     template <int SPAN>
     __device__ __forceinline__ void store_out(int b, int idx0, const float2* v) const
@@ -246,7 +247,7 @@ struct SignalIO {
         if (g0 + b >= count) return;
         float2* p = out + (g0 + b) * N + idx0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) p[kOutRun[u] * SPAN] = v[u];
+        for (int u = 0; u < 8; ++u) st_out<true>(&p[kOutRun[u] * SPAN], v[u]);
     }
 };
 

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(512) void fft_mfma_pass_kernel(const float2* in, fl
     for (int e = tid; e < T; e += WG) {
         const int r = e >> SH::LOGCB, b = e & (CB - 1);
         const long long j = j0 + b;
-        float2 v = src[j + r * stride];
+        float2 v = ld_in<true>(&src[j + r * stride]);   // read once
         if constexpr (!FIRST) {
             // W_N^{r j}: the transposed copy (columns side by side) when the context has it
             const float2 w = xtw ? xtw[((long long)r << (log_xtw - logn)) * (1 << kLogXtwCols) + j] : pyr[N + r * j];
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(512) void fft_mfma_pass_kernel(const float2* in, fl
 #pragma unroll 4
         for (int e = tid; e < T; e += WG) {
             const int k = e >> SH::LOGCB, b = e & (CB - 1);
-            dst[j0 + b + k * stride] = lds[tile_at<TL>(b, k)];
+            st_out<true>(&dst[j0 + b + k * stride], lds[tile_at<TL>(b, k)]);   // the transform's output
         }
     }
 }
