@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kPersistWG) void fft_persistent_kernel(const BatchI
                                                                      const BatchTransform* __restrict__ trs,
                                                                      int* __restrict__ ctl, DeviceTables tb)
 {
-    __shared__ float lds[kPersistLdsFloats];
+    __shared__ __align__(16) float lds[kPersistLdsFloats];   // 16-byte LDS reads of the pack staging
     __shared__ int s_item;
     persistent_body<false>(items, n_items, trs, ctl, tb, lds, s_item);
 }
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kPersistWG, 4) void fft_persistent_narrow_kernel(co
                                                                               const BatchTransform* __restrict__ trs,
                                                                               int* __restrict__ ctl, DeviceTables tb)
 {
-    __shared__ float lds[kNarrowLdsFloats];
+    __shared__ __align__(16) float lds[kNarrowLdsFloats];
     __shared__ int s_item;
     persistent_body<true>(items, n_items, trs, ctl, tb, lds, s_item);
 }
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kPersistWG, 4) void fft_persistent_narrow_kernel(co
 __global__ __launch_bounds__(kPersistWG) void fft_persistent_inline_kernel(InlineQueue q, int* __restrict__ ctl,
                                                                             DeviceTables tb)
 {
-    __shared__ float lds[kPersistLdsFloats];
+    __shared__ __align__(16) float lds[kPersistLdsFloats];   // 16-byte LDS reads of the pack staging
     __shared__ int s_item;
     persistent_body<false>(q.items, q.n_items, q.trs, ctl, tb, lds, s_item);
 }

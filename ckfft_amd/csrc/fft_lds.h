@@ -461,39 +461,46 @@ __device__ __forceinline__ void last_stage_pack_mirror(float2* v, int t, const f
             return;
         }
         float2* stg = reinterpret_cast<float2*>(stage);   // the exchange planes are free now
+        // points [c0, c0 + lead) share a line with the part before them; from c0 + lead on,
+        // every 64 points a wave stores are 4 whole lines (dst + c0 is equally far from a
+        // line in both halves: N/2 points are whole lines).  The staged half sits `sh`
+        // slots up so that the pairs read back from c0 + lead start on 16 bytes and load
+        // as single 16-byte LDS reads (8-byte-aligned pairs read as two halves that
+        // conflict on the banks: 247 conflict cycles per wave).
+        const unsigned mis = (unsigned)(((unsigned long long)dst) & 127u);
+        const int lead = mis ? (int)((128u - mis) >> 3) : 0;
+        const int sh = lead & 1;
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int c0 = half ? N / 2 : 0;
             const int c1 = half ? N + 1 : N / 2;
+            float2* sg = stg + sh - c0;                    // sg[k]: point k of this half
             // point t + rM lies in the upper half for r >= R/2; its mirror, for r < R/2
             if (t != 0 && active) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    if ((r >= R / 2) == (half == 1)) stg[t + r * M - c0] = v[r];
-                    else stg[(M - t) + (R - 1 - r) * M - c0] = v[R + R - 1 - r];
+                    if ((r >= R / 2) == (half == 1)) sg[t + r * M] = v[r];
+                    else sg[(M - t) + (R - 1 - r) * M] = v[R + R - 1 - r];
                 }
             }
-            if (t < 2 * R && active && (spk >= N / 2) == (half == 1)) stg[spk - c0] = sp;
-            if (half == 1 && t == 0 && active) stg[N - c0] = nyq;
+            if (t < 2 * R && active && (spk >= N / 2) == (half == 1)) sg[spk] = sp;
+            if (half == 1 && t == 0 && active) sg[N] = nyq;
             __syncthreads();
             if (active) {
-                // points [c0, a) share a line with the part before them; from a on, every
-                // 64 points a wave stores are 4 whole lines
-                const unsigned mis = (unsigned)(((unsigned long long)(dst + c0)) & 127u);
-                const int a = c0 + (mis ? (int)((128u - mis) >> 3) : 0);
-                if (t < a - c0) st_out<NT>(&dst[c0 + t], stg[t]);
+                const int a = c0 + lead;
+                if (t < lead) st_out<NT>(&dst[c0 + t], sg[c0 + t]);
                 const int cnt = c1 - a;
                 // pairs of points as 16-byte stores (dst + a is on a line): half the store
                 // instructions of one point per lane, forward 41.4 -> 40.7 us from HBM
                 float4* dst4 = reinterpret_cast<float4*>(dst + a);
+                const float4* sg4 = reinterpret_cast<const float4*>(sg + a);   // sh + lead is even
 #pragma unroll
                 for (int j = 0; j < (N / 2 + 1 + 2 * PL::T - 1) / (2 * PL::T); ++j) {
                     const int i = t + j * PL::T;
                     if (2 * i + 1 < cnt) {
-                        const float2 u0 = stg[a - c0 + 2 * i], u1 = stg[a - c0 + 2 * i + 1];
-                        st_out4<NT>(&dst4[i], make_float4(u0.x, u0.y, u1.x, u1.y));
+                        st_out4<NT>(&dst4[i], sg4[i]);
                     } else if (2 * i < cnt) {
-                        st_out<NT>(&dst[a + 2 * i], stg[a - c0 + 2 * i]);
+                        st_out<NT>(&dst[a + 2 * i], sg[a + 2 * i]);
                     }
                 }
             }
@@ -697,8 +704,9 @@ struct LdsShape {
     static_assert(!WIDE_EDGE || TPW == 1, "the wide pack/unpack keeps one transform's values");
     static constexpr int PART_OFFSET = SPLIT ? SPAN : 2 * SPAN * TPW;
     static constexpr int LDS_WORDS = PL::NSTAGES > 1 ? PART_OFFSET + (WIDE_EDGE ? 4 << EDGE_LOG : 0) : 1;
-    // the wide pack stages each half of the N+1 outputs (N + 2 floats) below PART_OFFSET
-    static_assert(!(WIDE_EDGE && KIND == kRealForward) || N + 2 <= PART_OFFSET, "pack staging fits the planes");
+    // the wide pack stages each half of the N+1 outputs (N + 2 floats, one slot up at most)
+    // below PART_OFFSET
+    static_assert(!(WIDE_EDGE && KIND == kRealForward) || N + 4 <= PART_OFFSET, "pack staging fits the planes");
     // complex elements per signal on each side
     static constexpr int IN_STRIDE = KIND == kRealInverse ? N + 1 : N;
     static constexpr int OUT_STRIDE = KIND == kRealForward ? N + 1 : N;

@@ -14,7 +14,7 @@ __global__ __launch_bounds__(WG) void fft_lds_kernel(const float2* __restrict__ 
                                                      const float2* __restrict__ tw, int count)
 {
     using SH = LdsShape<LOGN, LOGP, WG, KIND, SPLIT>;
-    __shared__ float lds[SH::LDS_WORDS];
+    __shared__ __align__(16) float lds[SH::LDS_WORDS];   // 16-byte LDS reads of the pack staging
     // one transform per workgroup: its index is uniform, so the signal's base address
     // stays in scalar registers
     const int tr = SH::TPW == 1 ? 0 : (int)threadIdx.x / SH::T;
