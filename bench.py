@@ -558,6 +558,7 @@ def huge_leg(torch, C, O, dev, reps: int = 10):
     return {"workload": "single complex forward, N=2^24, U[-1,1]", "ms": round(ms, 4),
             "GFLOP_s": round(5.0 * n * 24 / (ms * 1e-3) / 1e9, 1), "achieved_GBs": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "passes": 3,
+            "working_set_bytes": 3 * 8 * n,   # input, output and the intermediate
             "relL2_fp64": float(f"{err:.3e}")}
 
 
