@@ -174,10 +174,13 @@ __device__ __forceinline__ void wave_lds_sync()
 // butterflies and unpacks them in place with unpack_pair.  Thread 0 holds the
 // self-mirrored butterflies 0 and M/2, whose partners are other elements of the same
 // butterflies: element (0, r), point r*M, pairs with element (0, R-r) (with Y[N] for
-// r = 0), and element (1, q), point M/2 + q*M, with element (1, R-1-q).  Thread 0 puts
-// their 2R original values in `part` (LDS outside the exchange planes), the 2R lanes of
-// its wave unpack one element each there, and thread 0 takes the results back after the
-// in-place pass, so no lane needs extra registers and no lane unpacks 2R points alone.
+// r = 0), and element (1, q), point M/2 + q*M, with element (1, R-1-q); in every case the
+// partner of point k is point N - k.  The 2R lanes of thread 0's wave unpack one element
+// each: lane e reads its point and the partner from memory itself, first of all its reads,
+// puts the result in `part` (LDS outside the exchange planes), and thread 0 takes the 2R
+// results after the in-place pass.  Earlier, thread 0 wrote its 2R loaded values to
+// `part` for the lanes to read, so its wave waited for all of thread 0's reads and two
+// more LDS round trips before its own unpack: 2.7 us of the real 2^16 inverse.
 template <class PL, bool INV, bool NT>
 __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t, const float2* __restrict__ y,
                                                                const float2* __restrict__ tw, float* part)
@@ -185,40 +188,34 @@ __device__ __forceinline__ void first_stage_unpack_mirror_wide(float2* v, int t,
     constexpr int N = PL::N;
     constexpr int R = 1 << PL::rlog(0);
     constexpr int B = PL::P / R;
+    constexpr int M = N / R;
     static_assert(B == 2 && 2 * R == 64, "one mirror pair of butterflies per thread, 2R lanes in a wave");
     const float2* lvl = tw + 2 * N;
+    // lane e < 2R of wave 0: element e of thread 0's butterflies, its partner and twiddle
+    float2 own = make_float2(0.0f, 0.0f), partner = own, lane_w = own;
+    if (t < 2 * R) {
+        const int q = t & (R - 1);
+        const int k = t < R ? q * M : M / 2 + q * M;
+        own = y[k];
+        partner = y[N - k];
+        lane_w = t < R ? cmul(lvl[0], w64(q * (32 / R))) : half_step_twiddle<R>(R - 1 - q);
+    }
     const float2 wl = lvl[t];                           // W^t; W^(t + rM) = W^t exp(-2 pi i r/2R)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         v[r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, 0), r)]);
         v[R + r] = ld_in<NT>(&y[in_index<PL, 0>(bfly<PL, 0, true>(t, 1), r)]);
     }
-    float2* pp = reinterpret_cast<float2*>(part);
-    if (t < 2 * R) {                                    // thread 0's wave
-        if (t == 0) {
-#pragma unroll
-            for (int e = 0; e < 2 * R; ++e) pp[e] = v[e];
-        }
-        wave_lds_sync();
-        // lane t unpacks element t of thread 0's butterflies
-        const int q = t & (R - 1);
-        const float2 own = pp[t];
-        float2 res;
-        if (t < R) {
-            const float2 partner = t == 0 ? y[N] : pp[R - t];
-            res = unpack_one(own, partner, cmul(lvl[0], w64(q * (32 / R))));
-        } else {
-            res = unpack_one(own, pp[R + R - 1 - q], half_step_twiddle<R>(R - 1 - q));
-        }
-        wave_lds_sync();
-        pp[t] = res;
-        wave_lds_sync();
-    }
 #pragma unroll
     for (int r = 0; r < R; ++r) unpack_pair(v[r], v[R + R - 1 - r], cmul(wl, w64(r * (32 / R))));
-    if (t == 0) {
+    if (t < 2 * R) {                                    // thread 0's wave
+        float2* pp = reinterpret_cast<float2*>(part);
+        pp[t] = unpack_one(own, partner, lane_w);
+        wave_lds_sync();
+        if (t == 0) {
 #pragma unroll
-        for (int e = 0; e < 2 * R; ++e) v[e] = pp[e];
+            for (int e = 0; e < 2 * R; ++e) v[e] = pp[e];
+        }
     }
     butterfly_compute<PL, 0, INV, true>(v, t, tw, 0);
     butterfly_compute<PL, 0, INV, true>(v, t, tw, 1);
