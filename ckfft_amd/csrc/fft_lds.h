@@ -631,7 +631,7 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
     float2 zk[Q], zm[Q];
     if constexpr (!SPLIT) {
         lds_write<PL, S, 3>(v, t, re, im);
-        __syncthreads();
+        exchange_sync<PL>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
@@ -643,7 +643,7 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
         }
     } else {
         lds_write<PL, S, 1>(v, t, re, im);
-        __syncthreads();
+        exchange_sync<PL>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
@@ -652,9 +652,9 @@ __device__ __forceinline__ void pack_epilogue(float2* v, int t, float* re, float
                 zm[q].x = re[LY::pad((N - k) & (N - 1))];
             }
         }
-        __syncthreads();
+        exchange_sync<PL>();
         lds_write<PL, S, 2>(v, t, re, im);
-        __syncthreads();
+        exchange_sync<PL>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int k = t + q * T;
