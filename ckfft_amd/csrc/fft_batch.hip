@@ -606,10 +606,21 @@ int CkFftBatchPlanExecute(CkFftBatchPlan* plan, void* stream)
             if (!plan->graph_ready) {
                 hipStream_t cap = nullptr;
                 if (!batch_error(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream")) return 0;
-                bool ok = batch_error(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture") &&
-                          batch_error(launch_plan_kernels(plan, cap), "capture launch") &&
-                          batch_error(hipStreamEndCapture(cap, &plan->graph), "end capture") &&
-                          batch_error(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0), "instantiate");
+                bool ok = batch_error(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+                if (ok) {
+                    // the capture is ended also after a failed launch, so that the stream
+                    // leaves capture mode before it is destroyed
+                    const hipError_t le = launch_plan_kernels(plan, cap);
+                    hipGraph_t g = nullptr;
+                    const hipError_t ee = hipStreamEndCapture(cap, &g);
+                    ok = batch_error(le, "capture launch") && batch_error(ee, "end capture");
+                    if (ok)
+                        plan->graph = g;
+                    else if (g)
+                        (void)hipGraphDestroy(g);
+                    ok = ok && batch_error(hipGraphInstantiate(&plan->gexec, plan->graph, nullptr, nullptr, 0),
+                                           "instantiate");
+                }
                 (void)hipStreamDestroy(cap);
                 if (!ok) return 0;
                 plan->graph_ready = true;
