@@ -27,6 +27,22 @@ __global__ __launch_bounds__(WG) void fft_lds_kernel(const float2* __restrict__ 
                                                threadIdx.x);
 }
 
+// One transform per workgroup with the exchange one LDS plane at a time, held to MINW
+// waves per SIMD so that several workgroups share a CU: one workgroup's loads and stores
+// then overlap another's arithmetic.  The grid is exactly `count` workgroups.
+template <int LOGN, int LOGP, int KIND, int MINW>
+__global__ __launch_bounds__((1 << LOGN) >> LOGP, MINW) void fft_lds_shared_cu_kernel(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw, int count)
+{
+    constexpr int WG = (1 << LOGN) >> LOGP;
+    using SH = LdsShape<LOGN, LOGP, WG, KIND, true>;
+    __shared__ __align__(16) float lds[SH::LDS_WORDS];
+    (void)count;
+    const long long g = (long long)blockIdx.x;
+    lds_transform<LOGN, LOGP, WG, KIND, true>(in + g * SH::IN_STRIDE, out + g * SH::OUT_STRIDE, true, tw, lds,
+                                              threadIdx.x);
+}
+
 // ---- host-side dispatch ------------------------------------------------------------
 
 using KernelFn = void (*)(const float2*, float2*, const float2*, int);
@@ -40,13 +56,24 @@ struct LaunchCfg {
 template <int LOGN, int KIND>
 constexpr LaunchCfg make_cfg()
 {
-    // points per thread: whole transform for N <= 16, 16 up to 2^14, 64 at 2^15
-    // (512 threads x 64 points measured faster than 1024 x 32 on MI355X)
-    constexpr int LOGP = LOGN <= 4 ? LOGN : (LOGN <= 14 ? 4 : 6);
-    constexpr int T = (1 << LOGN) >> LOGP;
-    constexpr int WG = T >= 256 ? T : 256;
-    constexpr bool SPLIT = LOGN >= 15;
-    return LaunchCfg{fft_lds_kernel<LOGN, LOGP, WG, KIND, SPLIT>, WG, WG / T};
+    // points per thread: whole transform for N <= 16, 16 up to 2^13, 32 at 2^14, 64 at
+    // 2^15 (512 threads x 64 points measured faster than 1024 x 32 on MI355X).
+    // 2^13 and 2^14 exchange one LDS plane at a time (34 / 68 KiB), so that up to four /
+    // two workgroups share a CU: batches of 256 MiB per buffer took 98-112 us against
+    // 111-122 us (complex 2^13) and 109-124 us against 124-137 us (complex 2^14) with both
+    // planes in LDS and one or two workgroups per CU.  The radix plan, and so every bit of
+    // the result, is the same.
+    if constexpr (LOGN == 13) {
+        return LaunchCfg{fft_lds_shared_cu_kernel<13, 4, KIND, 6>, 512, 1};
+    } else if constexpr (LOGN == 14) {
+        return LaunchCfg{fft_lds_shared_cu_kernel<14, 5, KIND, 4>, 512, 1};
+    } else {
+        constexpr int LOGP = LOGN <= 4 ? LOGN : (LOGN <= 14 ? 4 : 6);
+        constexpr int T = (1 << LOGN) >> LOGP;
+        constexpr int WG = T >= 256 ? T : 256;
+        constexpr bool SPLIT = LOGN >= 15;
+        return LaunchCfg{fft_lds_kernel<LOGN, LOGP, WG, KIND, SPLIT>, WG, WG / T};
+    }
 }
 
 template <int KIND, int... L>
