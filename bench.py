@@ -66,9 +66,9 @@ def load_traffic(workload: str):
 def cpu_info():
     """This host: CPU model, the cores this process may run on (sched_getaffinity), the
     cgroup CPU quota if one is set, and the thread count the CPU baseline uses: every
-    affinity core, but no more than the CPU share (the cgroup quota, else the 16 host cores
-    a one-GPU box allots per GPU) -- threads beyond the share would only time-slice, or take
-    cores from other tenants of the node."""
+    affinity core, but no more than the CPU share (the cgroup quota, else 16 host cores, the
+    share of one GPU on a shared MI355X node) -- threads beyond the share would only
+    time-slice, or take cores from other tenants of the node."""
     model = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:

@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu on a GPU box")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu on a machine with one")
 
 
 @pytest.fixture(scope="session")
