@@ -120,6 +120,45 @@ def test_twiddle_table_is_fp64_rounded(ctx, C):
             assert buf[2 * (L // 4)] == 0.0 and buf[2 * (L // 4) + 1] == -1.0
 
 
+def test_fine_and_transposed_twiddle_tables(torch, C):
+    """The tables above the pyramid, read directly from a caller's arena (nMax = 2^28): the
+    fine tables (levels 2^17 .. 2^28, entries m < 2^12) within half an FP32 ulp of the
+    exact exp(-2 pi i m / L), and the transposed inter-pass table, table[r * 256 + j] =
+    pyramid level 2^16 entry r j mod 2^16, bit for bit.  The arena holds the pyramid (2 *
+    2^16 entries), then the fine tables, then the transposed table, each starting on 256
+    bytes (table_layout in ckfft_amd/csrc/ckfft_api.cpp; the sizes are checked against
+    CkFftDeviceTableBytes here)."""
+    n = 1 << 28
+    log_pyr, log_fine, nfine = 16, 12, 28 - 16
+    pyr_bytes = 8 * (2 << log_pyr)
+    fine_bytes = 8 * (nfine << log_fine)
+    xtw_bytes = 8 * (1 << log_pyr)
+    need = C.CkFftDeviceTableBytes(n, C.kCkFftDirection_Both)
+    assert need == pyr_bytes + fine_bytes + xtw_bytes, need
+    arena = torch.zeros(need, dtype=torch.uint8, device="cuda")
+    with C.Context(n, C.kCkFftDirection_Both, arena=arena) as ac:
+        torch.cuda.synchronize()
+        tab = arena.cpu().numpy().view(np.float32)
+        # the pyramid sits at offset 0: level 2^16 as CkFftGetTwiddles reports it
+        L = 1 << log_pyr
+        lvl = np.zeros(2 * L, np.float32)
+        assert ac.twiddles(L, lvl) == 1
+        assert np.array_equal(tab[2 * L:4 * L].view(np.uint32), lvl.view(np.uint32))
+        fine = tab[pyr_bytes // 4:(pyr_bytes + fine_bytes) // 4].reshape(nfine, 2 << log_fine)
+        m = np.arange(1 << log_fine)
+        for i in range(nfine):
+            Lf = 1 << (log_pyr + 1 + i)
+            exact = O.interleave(O.exact_twiddle(m, Lf, -1))
+            err = np.abs(fine[i].astype(np.float64) - exact)
+            assert float(err.max()) <= 2.0 ** -25, (Lf, float(err.max()))
+            assert np.mean(fine[i] != exact.astype(np.float32)) < 1e-3, Lf
+        xtw = tab[(pyr_bytes + fine_bytes) // 4:].view(np.uint64).reshape(256, 256)   # one complex per uint64
+        r = np.arange(256)[:, None]
+        j = np.arange(256)[None, :]
+        want = lvl.view(np.uint64)[(r * j) % L]
+        assert np.array_equal(xtw, want)
+
+
 # ------------------------------------------------------------------------- all sizes
 
 @pytest.mark.parametrize("kind", ["cfwd", "cinv"])
